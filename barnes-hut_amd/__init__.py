@@ -1,0 +1,288 @@
+"""ctypes binding of libbh.so, the MI355X Barnes-Hut engine (C ABI: include/bh.h).
+
+The package directory name carries a hyphen, so load it by path::
+
+    import importlib.util, pathlib
+    spec = importlib.util.spec_from_file_location(
+        "bh_amd", pathlib.Path("barnes-hut_amd/__init__.py"),
+        submodule_search_locations=[])
+    bh = importlib.util.module_from_spec(spec); spec.loader.exec_module(bh)
+
+`Engine` mirrors the reference's `--fc tree` step (src/main.cpp:191-281):
+`compute_accelerations` is the drop-in for bbox -> keys -> sort -> build ->
+exchange -> computeAccelerations (src/main.cpp:212-263) and `step` runs the
+drift-first kick-drift-kick loop.  Arrays are numpy (n, 3) float64 in the
+layout of `struct Vec3` (include/body.h:4-6).  Errors raise `BHError` with the
+library's message; nothing falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libbh.so"
+
+OK = 0
+ERR_ARG, ERR_NOMEM, ERR_HIP, ERR_STATE, ERR_IO, ERR_UNSUPPORTED, ERR_OVERFLOW = -1, -2, -3, -4, -5, -6, -7
+FC_TREE, FC_LET, FC_LET_DIRECT = 0, 1, 2
+FORM_PLAIN, FORM_MIXED, FORM_LEARNED = 0, 1, 2
+GEN_BELT, GEN_KUIPER, GEN_TROJAN = 0, 1, 2
+G = 1.48812e-34
+MAX_DEPTH = 21
+CENTRAL_MASS = 1.989e30
+
+
+class BHError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bh error {code}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int), ("fc_policy", C.c_int), ("form", C.c_int),
+                ("theta", C.c_double), ("G", C.c_double), ("soft2", C.c_double)]
+
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_D = C.c_double
+
+# (name, restype, argtypes) of every entry point declared in include/bh.h
+SIGNATURES = [
+    ("bh_config_default", None, [C.POINTER(Config)]),
+    ("bh_create", C.c_int, [C.POINTER(_P), C.POINTER(Config)]),
+    ("bh_destroy", None, [_P]),
+    ("bh_last_error", C.c_char_p, [_P]),
+    ("bh_set_bodies", C.c_int, [_P, _I64, _P, _P, _P, _P]),
+    ("bh_num_bodies", _I64, [_P]),
+    ("bh_get_bodies", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("bh_compute_accelerations", C.c_int, [_P, C.c_int, _D, _D, _D, _P]),
+    ("bh_step", C.c_int, [_P, _I64, _D]),
+    ("bh_time", _D, [_P]),
+    ("bh_count_steps", _I64, [_D, _D]),
+    ("bh_parse_time", C.c_int, [C.c_char_p, C.POINTER(_D)]),
+    ("bh_export_keys", C.c_int, [_P, _P]),
+    ("bh_export_bbox", C.c_int, [_P, _P]),
+    ("bh_tree_size", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64)]),
+    ("bh_export_topology", C.c_int, [_P, _P, _P, _P, _P]),
+    ("bh_set_counting", C.c_int, [_P, C.c_int]),
+    ("bh_get_counters", C.c_int, [_P, _P, _P, _P]),
+    ("bh_get_phase_ms", C.c_int, [_P, _P]),
+    ("bh_get_step_phase_ms", C.c_int, [_P, _P, C.POINTER(_I64)]),
+    ("bh_direct_sum", C.c_int, [_P, _I64, _P, _D, _D, _P]),
+    ("bh_generate", C.c_int, [C.c_int, C.c_uint64, _I64, _P, _P, _P]),
+    ("bh_generate_ensemble_sizes", C.c_int, [C.c_uint64, _I64, _I64, _I64, _P]),
+    ("bh_write_csv", C.c_int, [C.c_char_p, _I64, _P, _P, _P]),
+    ("bh_read_csv", C.c_int, [C.c_char_p, _I64, _I64, C.POINTER(_I64), _P, _P, _P, _P]),
+]
+
+_lib = None
+
+
+def load(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Load libbh.so (built in-tree by `make -C barnes-hut_amd`)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise BHError(ERR_STATE, f"{p} is not built (run __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(lib, rc: int, ctx=None):
+    if rc != OK:
+        msg = lib.bh_last_error(ctx)
+        raise BHError(rc, msg.decode() if msg else "")
+
+
+def generate(kind: int, seed: int, n: int):
+    """Synthetic system: returns pos (n,3), vel (n,3), mass (n,)."""
+    lib = load()
+    pos = np.zeros((n, 3))
+    vel = np.zeros((n, 3))
+    mass = np.zeros(n)
+    rc = lib.bh_generate(kind, seed, n, _ptr(pos), _ptr(vel), _ptr(mass))
+    if rc != OK:
+        raise BHError(rc, f"bh_generate(kind={kind}, seed={seed}, n={n})")
+    return pos, vel, mass
+
+
+def ensemble_sizes(seed: int, count: int, nmin: int, nmax: int) -> np.ndarray:
+    lib = load()
+    out = np.zeros(count, np.int64)
+    rc = lib.bh_generate_ensemble_sizes(seed, count, nmin, nmax, _ptr(out))
+    if rc != OK:
+        raise BHError(rc, "bh_generate_ensemble_sizes")
+    return out
+
+
+def write_csv(path, pos, vel, mass):
+    lib = load()
+    pos, vel, mass = (np.ascontiguousarray(a, np.float64) for a in (pos, vel, mass))
+    rc = lib.bh_write_csv(str(path).encode(), len(mass), _ptr(pos), _ptr(vel), _ptr(mass))
+    if rc != OK:
+        raise BHError(rc, f"bh_write_csv({path})")
+
+
+def read_csv(path, body_limit: int = -1):
+    """Returns ids, mass, pos, vel like readCSV (src/io.cpp:16-81)."""
+    lib = load()
+    n = _I64(0)
+    rc = lib.bh_read_csv(str(path).encode(), body_limit, 0, C.byref(n), None, None, None, None)
+    if rc != OK:
+        raise BHError(rc, f"bh_read_csv({path})")
+    cnt = n.value
+    pos = np.zeros((cnt, 3))
+    vel = np.zeros((cnt, 3))
+    mass = np.zeros(cnt)
+    ids = np.zeros(cnt, np.uint64)
+    rc = lib.bh_read_csv(str(path).encode(), body_limit, cnt, C.byref(n), _ptr(pos), _ptr(vel), _ptr(mass), _ptr(ids))
+    if rc != OK:
+        raise BHError(rc, f"bh_read_csv({path})")
+    return ids, mass, pos, vel
+
+
+def count_steps(dt: float, t_end: float) -> int:
+    return load().bh_count_steps(dt, t_end)
+
+
+def parse_time(text: str) -> float:
+    out = _D(0.0)
+    rc = load().bh_parse_time(text.encode(), C.byref(out))
+    if rc != OK:
+        raise BHError(rc, f"cannot parse time {text!r}")
+    return out.value
+
+
+class Engine:
+    """One device context (bh_ctx)."""
+
+    def __init__(self, device: int = 0, form: int = FORM_PLAIN, theta: float = 1.05,
+                 G_: float = G, soft2: float = 0.0, fc_policy: int = FC_TREE):
+        self.lib = load()
+        cfg = Config()
+        self.lib.bh_config_default(C.byref(cfg))
+        cfg.device, cfg.form, cfg.theta, cfg.G, cfg.soft2, cfg.fc_policy = device, form, theta, G_, soft2, fc_policy
+        h = _P()
+        rc = self.lib.bh_create(C.byref(h), C.byref(cfg))
+        if rc != OK:
+            raise BHError(rc, self.lib.bh_last_error(None).decode())
+        self.h = h
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc):
+        _check(self.lib, rc, self.h)
+
+    def set_bodies(self, pos, vel, mass, ids=None):
+        pos = np.ascontiguousarray(pos, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        mass = np.ascontiguousarray(mass, np.float64)
+        idp = None if ids is None else np.ascontiguousarray(ids, np.uint64)
+        self.n = len(mass)
+        self._ck(self.lib.bh_set_bodies(self.h, self.n, _ptr(pos), _ptr(vel), _ptr(mass), _ptr(idp)))
+
+    def get_bodies(self):
+        n = self.n
+        pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        mass, ids = np.zeros(n), np.zeros(n, np.uint64)
+        self._ck(self.lib.bh_get_bodies(self.h, _ptr(pos), _ptr(vel), _ptr(mass), _ptr(ids), _ptr(acc)))
+        return dict(pos=pos, vel=vel, mass=mass, ids=ids, acc=acc)
+
+    def compute_accelerations(self, form: int = FORM_PLAIN, theta: float = 1.05, G_: float = G,
+                              soft2: float = 0.0) -> np.ndarray:
+        acc = np.zeros((self.n, 3))
+        self._ck(self.lib.bh_compute_accelerations(self.h, form, theta, G_, soft2, _ptr(acc)))
+        return acc
+
+    def step(self, nsteps: int, dt: float):
+        self._ck(self.lib.bh_step(self.h, nsteps, dt))
+
+    @property
+    def time(self) -> float:
+        return self.lib.bh_time(self.h)
+
+    def keys(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint64)
+        self._ck(self.lib.bh_export_keys(self.h, _ptr(out)))
+        return out
+
+    def bbox(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._ck(self.lib.bh_export_bbox(self.h, _ptr(out)))
+        return out
+
+    def tree_size(self):
+        a, b = _I64(0), _I64(0)
+        self._ck(self.lib.bh_tree_size(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def topology(self):
+        nf, _ = self.tree_size()
+        prefix = np.zeros(nf, np.uint64)
+        depth = np.zeros(nf, np.uint8)
+        mass = np.zeros(nf)
+        com = np.zeros((nf, 3))
+        self._ck(self.lib.bh_export_topology(self.h, _ptr(prefix), _ptr(depth), _ptr(mass), _ptr(com)))
+        return prefix, depth, mass, com
+
+    def set_counting(self, on: bool):
+        self._ck(self.lib.bh_set_counting(self.h, 1 if on else 0))
+
+    def counters(self):
+        """(accepted, opened full-chain, tested compressed) per body."""
+        a = np.zeros(self.n, np.int64)
+        o = np.zeros(self.n, np.int64)
+        t = np.zeros(self.n, np.int64)
+        self._ck(self.lib.bh_get_counters(self.h, _ptr(a), _ptr(o), _ptr(t)))
+        return a, o, t
+
+    def phase_ms(self) -> np.ndarray:
+        out = np.zeros(5)
+        self._ck(self.lib.bh_get_phase_ms(self.h, _ptr(out)))
+        return out
+
+    def step_phase_ms(self):
+        out = np.zeros(5)
+        k = _I64(0)
+        self._ck(self.lib.bh_get_step_phase_ms(self.h, _ptr(out), C.byref(k)))
+        return out, k.value
+
+    def direct_sum(self, idx, G_: float = G, soft2: float = 0.0) -> np.ndarray:
+        """FP64 direct-sum accelerations (device) for bodies idx of the current order."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        out = np.zeros((len(idx), 3))
+        self._ck(self.lib.bh_direct_sum(self.h, len(idx), _ptr(idx), G_, soft2, _ptr(out)))
+        return out

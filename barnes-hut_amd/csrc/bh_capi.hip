@@ -1,0 +1,582 @@
+// bh_capi.hip -- the C ABI of include/bh.h: context, device memory, the
+// force evaluation pipeline and the KDK step loop.
+//
+// One force evaluation is a fixed sequence of kernels on the context's
+// stream with no host round trip:
+//   kick/drift/box -> keys -> radix sort -> gather -> tree -> walk (+ kick)
+// so a step costs one launch sequence and the host only waits when the
+// caller asks for data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bh.h"
+#include "bh_internal.hpp"
+
+using bh::Bodies;
+using bh::StepScalars;
+using bh::TreeBuffers;
+using bh::WalkNode;
+
+struct bh_ctx {
+    bh_config cfg{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t n = 0, cap = 0;
+    double t = 0.0;
+    Bodies cur{}, alt{};
+    TreeBuffers tb{};
+    double* partials = nullptr;
+    int64_t* cnt_a = nullptr;
+    int64_t* cnt_o = nullptr;
+    int64_t* cnt_t = nullptr;
+    bool counting = false;
+    bool have_tree = false;
+    bool have_counts = false;
+    hipEvent_t ev[5] = {};
+    // per-step phase events of bh_step: ring of kRing sets of 5
+    std::vector<hipEvent_t> ring;
+    int64_t ring_steps = 0;       // steps recorded by the last bh_step call
+    const hipEvent_t* last_ev = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+std::string g_create_error;
+
+int fail(bh_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_create_error = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), BH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+    } while (0)
+
+void free_buffers(bh_ctx* c)
+{
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    c->cur = Bodies{};
+    c->alt = Bodies{};
+    c->tb = TreeBuffers{};
+    c->partials = nullptr;
+    c->cnt_a = c->cnt_o = c->cnt_t = nullptr;
+    c->cap = 0;
+    c->have_tree = false;
+    c->have_counts = false;
+}
+
+template <class T>
+int dalloc(bh_ctx* c, T** p, size_t count)
+{
+    void* q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, BH_ERR_NOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return BH_OK;
+}
+
+int reserve(bh_ctx* c, size_t n)
+{
+    if (n <= c->cap && c->cap > 0) return BH_OK;
+    free_buffers(c);
+    const size_t N = std::max<size_t>(n, 64);
+    int rc = BH_OK;
+#define A(p, cnt) \
+    if ((rc = dalloc(c, &(p), (cnt))) != BH_OK) return rc
+    for (Bodies* b : {&c->cur, &c->alt}) {
+        A(b->px, N); A(b->py, N); A(b->pz, N);
+        A(b->vx, N); A(b->vy, N); A(b->vz, N);
+        A(b->m, N); A(b->id, N);
+    }
+    A(c->cur.ax, N); A(c->cur.ay, N); A(c->cur.az, N);
+    c->alt.ax = c->cur.ax; c->alt.ay = c->cur.ay; c->alt.az = c->cur.az;
+    TreeBuffers& t = c->tb;
+    A(t.key, N); A(t.key_alt, N); A(t.perm, N); A(t.perm_alt, N);
+    A(t.flag, N); A(t.leaf_of, N); A(t.self_w, N);
+    A(t.ukey, N); A(t.leaf_first, N + 1); A(t.delta, N); A(t.rep, N); A(t.is_rep, N);
+    A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
+    A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
+    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.walk, 2 * N);
+    A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
+    A(t.sc, 1);
+    A(c->partials, 6 * 1024);
+#undef A
+    HIP_TRY(c, hipMemsetAsync(t.sc, 0, sizeof(StepScalars), c->stream));
+    c->cap = N;
+    return BH_OK;
+}
+
+// One force evaluation with optional leading kick + drift and trailing kick.
+constexpr int kRing = 512;
+
+int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool kick_before, bool drift,
+              double dt, bool kick_after, const hipEvent_t* ev)
+{
+    if (form != BH_FORM_PLAIN)
+        return fail(c, BH_ERR_UNSUPPORTED, "form %d is not available in this build", form);
+    if (c->n == 0) return BH_OK;
+    hipStream_t s = c->stream;
+    const double half_dt = 0.5 * dt;
+    HIP_TRY(c, hipEventRecord(ev[0], s));
+    bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb.sc, c->partials, s);
+    bh::launch_keys(c->cur, c->n, c->tb.sc, c->tb.key, c->tb.perm, s);
+    HIP_TRY(c, hipEventRecord(ev[1], s));
+    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63, c->tb.sort_tmp, s);
+    bh::launch_permute(c->cur, c->alt, c->tb.perm, c->tb.key, c->tb.flag, c->n, s);
+    std::swap(c->cur, c->alt);
+    HIP_TRY(c, hipEventRecord(ev[2], s));
+    bh::launch_build_tree(c->cur, c->n, c->tb, s);
+    HIP_TRY(c, hipEventRecord(ev[3], s));
+    bh::WalkParams p{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0};
+    const bool cnt = c->counting;
+    if (cnt && !c->cnt_a) {
+        int rc = dalloc(c, &c->cnt_a, c->cap);
+        if (rc) return rc;
+        if ((rc = dalloc(c, &c->cnt_o, c->cap))) return rc;
+        if ((rc = dalloc(c, &c->cnt_t, c->cap))) return rc;
+    }
+    bh::launch_walk_plain(c->cur, c->n, c->tb, p, cnt ? c->cnt_a : nullptr, cnt ? c->cnt_o : nullptr,
+                          cnt ? c->cnt_t : nullptr, s);
+    HIP_TRY(c, hipEventRecord(ev[4], s));
+    HIP_TRY(c, hipGetLastError());
+    c->last_ev = ev;
+    c->have_tree = true;
+    c->have_counts = cnt;
+    return BH_OK;
+}
+
+int check_ctx(const bh_ctx* c) { return c ? BH_OK : BH_ERR_ARG; }
+
+void to_soa(const bh_vec3* v, size_t n, std::vector<double>& x, std::vector<double>& y, std::vector<double>& z)
+{
+    x.resize(n); y.resize(n); z.resize(n);
+    for (size_t i = 0; i < n; ++i) { x[i] = v[i].x; y[i] = v[i].y; z[i] = v[i].z; }
+}
+
+int download3(bh_ctx* c, bh_vec3* out, const double* dx, const double* dy, const double* dz)
+{
+    if (!out) return BH_OK;
+    std::vector<double> x(c->n), y(c->n), z(c->n);
+    HIP_TRY(c, hipMemcpyAsync(x.data(), dx, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(y.data(), dy, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(z.data(), dz, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->n; ++i) out[i] = bh_vec3{x[i], y[i], z[i]};
+    return BH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void bh_config_default(bh_config* cfg)
+{
+    if (!cfg) return;
+    cfg->device = 0;
+    cfg->fc_policy = BH_FC_TREE;
+    cfg->form = BH_FORM_PLAIN;
+    cfg->theta = 1.05;
+    cfg->G = BH_G;
+    cfg->soft2 = 0.0;
+}
+
+int bh_create(bh_ctx** out, const bh_config* cfg)
+{
+    if (!out) return fail(nullptr, BH_ERR_ARG, "bh_create: out is NULL");
+    *out = nullptr;
+    bh_config c0;
+    bh_config_default(&c0);
+    if (cfg) c0 = *cfg;
+    if (c0.fc_policy != BH_FC_TREE)
+        return fail(nullptr, BH_ERR_UNSUPPORTED, "fc policy %d: one GPU holds every body, only the full tree exists",
+                    c0.fc_policy);
+    if (c0.form < BH_FORM_PLAIN || c0.form > BH_FORM_LEARNED)
+        return fail(nullptr, BH_ERR_ARG, "unknown form %d", c0.form);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(nullptr, BH_ERR_HIP, "no HIP device: %s", e != hipSuccess ? hipGetErrorString(e) : "count is 0");
+    if (c0.device < 0 || c0.device >= ndev) return fail(nullptr, BH_ERR_ARG, "device %d of %d", c0.device, ndev);
+    e = hipSetDevice(c0.device);
+    if (e != hipSuccess) return fail(nullptr, BH_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    bh_ctx* c = new bh_ctx();
+    c->cfg = c0;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, BH_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (auto& ev : c->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            bh_destroy(c);
+            return fail(nullptr, BH_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
+    *out = c;
+    return BH_OK;
+}
+
+void bh_destroy(bh_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_buffers(c);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : c->ring)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* bh_last_error(const bh_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, const double* mass,
+                  const uint64_t* ids)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (n < 0 || n > (int64_t{1} << 30)) return fail(c, BH_ERR_ARG, "body count %lld out of range", (long long)n);
+    if (n > 0 && (!pos || !vel || !mass)) return fail(c, BH_ERR_ARG, "pos, vel and mass are required");
+    HIP_TRY(c, hipSetDevice(c->cfg.device));
+    int rc = reserve(c, static_cast<size_t>(n));
+    if (rc) return rc;
+    c->n = static_cast<size_t>(n);
+    c->t = 0.0;
+    c->have_tree = false;
+    c->have_counts = false;
+    if (n == 0) return BH_OK;
+    std::vector<double> x, y, z;
+    hipStream_t s = c->stream;
+    to_soa(pos, c->n, x, y, z);
+    HIP_TRY(c, hipMemcpyAsync(c->cur.px, x.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->cur.py, y.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->cur.pz, z.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    to_soa(vel, c->n, x, y, z);
+    HIP_TRY(c, hipMemcpyAsync(c->cur.vx, x.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->cur.vy, y.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->cur.vz, z.data(), n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->cur.m, mass, n * 8, hipMemcpyHostToDevice, s));
+    std::vector<uint64_t> idv;
+    if (!ids) {
+        idv.resize(c->n);
+        for (size_t i = 0; i < c->n; ++i) idv[i] = i;
+        ids = idv.data();
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->cur.id, ids, n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemsetAsync(c->cur.ax, 0, n * 8, s));
+    HIP_TRY(c, hipMemsetAsync(c->cur.ay, 0, n * 8, s));
+    HIP_TRY(c, hipMemsetAsync(c->cur.az, 0, n * 8, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    return BH_OK;
+}
+
+int64_t bh_num_bodies(const bh_ctx* c) { return c ? static_cast<int64_t>(c->n) : -1; }
+
+int bh_get_bodies(bh_ctx* c, bh_vec3* pos, bh_vec3* vel, double* mass, uint64_t* ids, bh_vec3* acc)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (c->n == 0) return BH_OK;
+    int rc;
+    if ((rc = download3(c, pos, c->cur.px, c->cur.py, c->cur.pz))) return rc;
+    if ((rc = download3(c, vel, c->cur.vx, c->cur.vy, c->cur.vz))) return rc;
+    if ((rc = download3(c, acc, c->cur.ax, c->cur.ay, c->cur.az))) return rc;
+    if (mass) HIP_TRY(c, hipMemcpy(mass, c->cur.m, c->n * 8, hipMemcpyDeviceToHost));
+    if (ids) HIP_TRY(c, hipMemcpy(ids, c->cur.id, c->n * 8, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+int bh_compute_accelerations(bh_ctx* c, int form, double theta, double G, double soft2, bh_vec3* acc_out)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (!(theta >= 0.0)) return fail(c, BH_ERR_ARG, "theta must be >= 0");
+    int rc = run_force(c, form, theta, G, soft2, false, false, 0.0, false, c->ev);
+    if (rc) return rc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return download3(c, acc_out, c->cur.ax, c->cur.ay, c->cur.az);
+}
+
+int bh_step(bh_ctx* c, int64_t nsteps, double dt)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (nsteps < 0) return fail(c, BH_ERR_ARG, "negative step count");
+    if (c->ring.empty()) {
+        c->ring.resize(5 * kRing, nullptr);
+        for (auto& e : c->ring) HIP_TRY(c, hipEventCreate(&e));
+    }
+    c->ring_steps = 0;
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const hipEvent_t* ev = c->ring.data() + 5 * (k % kRing);
+        int rc = run_force(c, c->cfg.form, c->cfg.theta, c->cfg.G, c->cfg.soft2, c->t > 0, true, dt, true, ev);
+        if (rc) return rc;
+        c->t += dt;
+        c->ring_steps = k + 1;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return BH_OK;
+}
+
+double bh_time(const bh_ctx* c) { return c ? c->t : 0.0; }
+
+int64_t bh_count_steps(double dt, double t_end)
+{
+    if (!(dt > 0.0)) return -1;
+    int64_t k = 0;
+    for (double t = 0.0; t < t_end; t += dt) ++k;
+    return k;
+}
+
+int bh_parse_time(const char* text, double* days)
+{
+    if (!text || !days) return BH_ERR_ARG;
+    const size_t len = std::strlen(text);
+    if (len < 2) return BH_ERR_ARG;
+    std::string num(text, len - 1);
+    char* end = nullptr;
+    const double v = std::strtod(num.c_str(), &end);
+    if (end == num.c_str()) return BH_ERR_ARG;
+    switch (text[len - 1]) {
+    case 'h': *days = v / 24.0; return BH_OK;
+    case 'd': *days = v; return BH_OK;
+    case 'y': *days = v * 365.25; return BH_OK;
+    default: return BH_ERR_ARG;
+    }
+}
+
+int bh_export_keys(bh_ctx* c, uint64_t* keys)
+{
+    if (check_ctx(c) || !keys) return BH_ERR_ARG;
+    if (!c->have_tree) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(keys, c->tb.key, c->n * 8, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+int bh_export_bbox(bh_ctx* c, double* bb6)
+{
+    if (check_ctx(c) || !bb6) return BH_ERR_ARG;
+    if (!c->have_tree) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    std::memcpy(bb6, sc.bb, sizeof sc.bb);
+    return BH_OK;
+}
+
+namespace {
+
+struct HostTree {
+    uint32_t M = 0, B = 0;
+    std::vector<uint64_t> ukey;
+    std::vector<uint8_t> bdepth;
+    std::vector<uint32_t> blo, parent, windex;
+    std::vector<WalkNode> walk;
+};
+
+int fetch_tree(bh_ctx* c, HostTree& h)
+{
+    if (!c->have_tree) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    if (sc.error) return fail(c, BH_ERR_STATE, "device tree check failed (error bits 0x%x)", sc.error);
+    h.M = sc.n_leaves;
+    h.B = sc.n_branch;
+    const size_t T = static_cast<size_t>(h.M) + h.B;
+    if (sc.n_nodes != T) return fail(c, BH_ERR_STATE, "tree size mismatch: %u nodes vs %zu", sc.n_nodes, T);
+    h.ukey.resize(h.M);
+    h.bdepth.resize(h.B);
+    h.blo.resize(h.B);
+    h.parent.resize(T);
+    h.windex.resize(T);
+    h.walk.resize(T);
+    HIP_TRY(c, hipMemcpy(h.ukey.data(), c->tb.ukey, h.M * 8ull, hipMemcpyDeviceToHost));
+    if (h.B) {
+        HIP_TRY(c, hipMemcpy(h.bdepth.data(), c->tb.bdepth, h.B, hipMemcpyDeviceToHost));
+        HIP_TRY(c, hipMemcpy(h.blo.data(), c->tb.blo, h.B * 4ull, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(c, hipMemcpy(h.parent.data(), c->tb.parent, T * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(h.windex.data(), c->tb.windex, T * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(h.walk.data(), c->tb.walk, T * sizeof(WalkNode), hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+inline uint64_t host_prefix(uint64_t code, int d) { return d == 0 ? 0 : (code & (~0ull << (63 - 3 * d))); }
+
+}  // namespace
+
+int bh_tree_size(bh_ctx* c, int64_t* n_full, int64_t* n_compressed)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    HostTree h;
+    int rc = fetch_tree(c, h);
+    if (rc) return rc;
+    int64_t full = 0;
+    const size_t T = static_cast<size_t>(h.M) + h.B;
+    for (size_t t = 0; t < T; ++t) {
+        const int bottom = t < h.M ? BH_MAX_DEPTH : h.bdepth[t - h.M];
+        const int top = h.parent[t] == 0xffffffffu ? 0 : h.bdepth[h.parent[t] - h.M] + 1;
+        full += bottom - top + 1;
+    }
+    if (n_full) *n_full = full;
+    if (n_compressed) *n_compressed = static_cast<int64_t>(T);
+    return BH_OK;
+}
+
+int bh_export_topology(bh_ctx* c, uint64_t* prefix, uint8_t* depth, double* mass, double* com)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    HostTree h;
+    int rc = fetch_tree(c, h);
+    if (rc) return rc;
+    struct Rec {
+        uint64_t p;
+        uint8_t d;
+        uint32_t w;
+    };
+    std::vector<Rec> v;
+    const size_t T = static_cast<size_t>(h.M) + h.B;
+    for (size_t t = 0; t < T; ++t) {
+        const bool leaf = t < h.M;
+        const int bottom = leaf ? BH_MAX_DEPTH : h.bdepth[t - h.M];
+        const int top = h.parent[t] == 0xffffffffu ? 0 : h.bdepth[h.parent[t] - h.M] + 1;
+        const uint64_t key = h.ukey[leaf ? t : h.blo[t - h.M]];
+        for (int d = top; d <= bottom; ++d) v.push_back(Rec{host_prefix(key, d), static_cast<uint8_t>(d), h.windex[t]});
+    }
+    std::sort(v.begin(), v.end(), [](const Rec& a, const Rec& b) {
+        if (a.d != b.d) return a.d < b.d;
+        return a.p < b.p;
+    });
+    for (size_t k = 0; k < v.size(); ++k) {
+        const WalkNode& nd = h.walk[v[k].w];
+        if (prefix) prefix[k] = v[k].p;
+        if (depth) depth[k] = v[k].d;
+        if (mass) mass[k] = nd.m;
+        if (com) {
+            com[3 * k] = nd.x;
+            com[3 * k + 1] = nd.y;
+            com[3 * k + 2] = nd.z;
+        }
+    }
+    return BH_OK;
+}
+
+int bh_set_counting(bh_ctx* c, int on)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->counting = on != 0;
+    return BH_OK;
+}
+
+int bh_get_counters(bh_ctx* c, int64_t* accepted, int64_t* opened, int64_t* tested)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (!c->have_counts) return fail(c, BH_ERR_STATE, "no counted walk yet (bh_set_counting)");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (accepted) HIP_TRY(c, hipMemcpy(accepted, c->cnt_a, c->n * 8, hipMemcpyDeviceToHost));
+    if (opened) HIP_TRY(c, hipMemcpy(opened, c->cnt_o, c->n * 8, hipMemcpyDeviceToHost));
+    if (tested) HIP_TRY(c, hipMemcpy(tested, c->cnt_t, c->n * 8, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+namespace {
+int phase_ms(bh_ctx* c, const hipEvent_t* ev, double* ms5)
+{
+    float f;
+    for (int k = 0; k < 4; ++k) {
+        HIP_TRY(c, hipEventElapsedTime(&f, ev[k], ev[k + 1]));
+        ms5[k] = f;
+    }
+    HIP_TRY(c, hipEventElapsedTime(&f, ev[0], ev[4]));
+    ms5[4] = f;
+    return BH_OK;
+}
+}  // namespace
+
+int bh_get_phase_ms(bh_ctx* c, double* ms5)
+{
+    if (check_ctx(c) || !ms5) return BH_ERR_ARG;
+    if (!c->have_tree || !c->last_ev) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return phase_ms(c, c->last_ev, ms5);
+}
+
+int bh_get_step_phase_ms(bh_ctx* c, double* ms5, int64_t* nsteps)
+{
+    if (check_ctx(c) || !ms5) return BH_ERR_ARG;
+    if (c->ring_steps == 0) return fail(c, BH_ERR_STATE, "no bh_step call yet");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const int64_t k0 = c->ring_steps > kRing ? c->ring_steps - kRing : 0;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int64_t k = k0; k < c->ring_steps; ++k) {
+        double ms[5];
+        int rc = phase_ms(c, c->ring.data() + 5 * (k % kRing), ms);
+        if (rc) return rc;
+        for (int q = 0; q < 5; ++q) acc[q] += ms[q];
+    }
+    const int64_t cnt = c->ring_steps - k0;
+    for (int q = 0; q < 5; ++q) ms5[q] = acc[q] / static_cast<double>(cnt);
+    if (nsteps) *nsteps = cnt;
+    return BH_OK;
+}
+
+int bh_direct_sum(bh_ctx* c, int64_t m, const int64_t* idx, double G, double soft2, bh_vec3* acc_out)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (m < 0 || (m > 0 && (!idx || !acc_out))) return fail(c, BH_ERR_ARG, "bad sample arguments");
+    if (m == 0) return BH_OK;
+    for (int64_t k = 0; k < m; ++k)
+        if (idx[k] < 0 || idx[k] >= static_cast<int64_t>(c->n))
+            return fail(c, BH_ERR_ARG, "sample index %lld out of range", (long long)idx[k]);
+    int64_t* d_idx = nullptr;
+    double* d_acc = nullptr;
+    HIP_TRY(c, hipMalloc(&d_idx, m * 8));
+    hipError_t e = hipMalloc(&d_acc, m * 24);
+    if (e != hipSuccess) {
+        (void)hipFree(d_idx);
+        return fail(c, BH_ERR_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
+    }
+    int rc = BH_OK;
+    e = hipMemcpyAsync(d_idx, idx, m * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        bh::launch_direct_sum(c->cur, c->n, d_idx, static_cast<size_t>(m), G, soft2, d_acc, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(acc_out, d_acc, m * 24, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = fail(c, BH_ERR_HIP, "direct sum: %s", hipGetErrorString(e));
+    (void)hipFree(d_idx);
+    (void)hipFree(d_acc);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// bh_internal.hpp -- device data layout and kernel launchers of libbh.so.
+//
+// Layout in HBM (all structure-of-arrays, FP64 state):
+//   bodies  px py pz vx vy vz ax ay az m  (double[N] each), id (u64[N]),
+//           key (u64[N]); a second set of the permuted arrays is the target of
+//           the Morton gather, after which the two sets swap.
+//   tree    distinct leaf keys u[M], per-leaf body start, boundary depths
+//           delta[M-1], branching nodes (depth, key range, up to 8 children),
+//           raw mass moments, and the walk array: one 48-byte WalkNode per
+//           compressed node in walk order (pre-order, children visited from
+//           octant 7 down to 0) with a skip index to the node that follows its
+//           subtree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace bh {
+
+constexpr int kMaxDepth = 21;
+
+// One compressed octree node in walk order.  A node stands for a chain of the
+// reference's map nodes (every unary node down to the chain bottom); all of
+// them hold the same mass and centre of mass, so the walk tests the chain at
+// its bottom depth, where the opening test side^2 < theta^2 r^2 is weakest.
+struct WalkNode {
+    double x, y, z;   // centre of mass after the exchange round trip
+    double m;         // mass
+    double side2;     // (L / 2^d)^2 at the test depth d (leaf: depth 20)
+    uint32_t skip;    // walk index of the first node after this subtree
+    uint32_t meta;    // kMeta* bits
+};
+static_assert(sizeof(WalkNode) == 48, "WalkNode is 48 bytes");
+
+constexpr uint32_t kMetaDepthMask = 31u;      // test (bottom) depth
+constexpr uint32_t kMetaTopShift = 5;         // chain top depth (5 bits)
+constexpr uint32_t kMetaLeaf = 1u << 10;      // distinct-key leaf (depth 21)
+constexpr uint32_t kMetaChain = 1u << 11;     // leaf chain starts above depth 21
+
+// Device-side scalars shared by the kernels of one force evaluation.
+struct StepScalars {
+    double bb[6];        // box min xyz, max xyz
+    double span[3];      // per-axis normalisation span (1.0 when degenerate)
+    double side2[22];    // (L / 2^d)^2, L = largest box extent
+    uint32_t n_leaves;   // M: distinct keys
+    uint32_t n_branch;   // B: branching nodes
+    uint32_t n_nodes;    // M + B
+    uint32_t root;       // unified id of the root (leaf ids 0..M-1, branch M..)
+    uint64_t n_full;     // node count of the reference's full-chain map
+    uint32_t block_counter;  // last-block-done counter, left at 0
+    uint32_t error;          // kErr* bits raised by bounds guards
+};
+
+constexpr uint32_t kErrWalkIndex = 1u;   // a walk index fell outside [0, M+B)
+
+// Scratch sizing helpers
+inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ----------------------------------------------------------------- scan
+// Exclusive prefix sum of n u32 values; total (optional device pointer)
+// receives the sum.  tmp must hold scan_tmp_words(n) u32.
+size_t scan_tmp_words(size_t n);
+void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
+                        hipStream_t s);
+
+// ----------------------------------------------------------------- sort
+// Stable LSD radix sort of (key, idx) pairs by the low `bits` key bits.
+// Input in (k0, v0); result in (k0, v0) after the call (ping-pong through
+// k1, v1).  tmp must hold sort_tmp_words(n) u32.
+size_t sort_tmp_words(size_t n);
+void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
+                      uint32_t* tmp, hipStream_t s);
+
+// ----------------------------------------------------------------- step
+struct Bodies {
+    double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *m;
+    uint64_t* id;
+};
+
+struct TreeBuffers {
+    // per body
+    uint64_t* key;        // sorted keys
+    uint64_t* key_alt;
+    uint32_t* perm;       // sort payload (gather index)
+    uint32_t* perm_alt;
+    uint32_t* flag;       // first-of-key flags, then leaf index (exclusive scan)
+    uint32_t* leaf_of;    // leaf id of each body
+    uint32_t* self_w;     // walk index of each body's leaf
+    // per leaf / boundary (capacity N)
+    uint64_t* ukey;       // distinct keys
+    uint32_t* leaf_first; // first body of each leaf (M+1 entries)
+    uint8_t* delta;       // split depth of boundary j (u[j], u[j+1])
+    uint32_t* rep;        // representative boundary of boundary j's node
+    uint32_t* is_rep;     // 1 for representative boundaries, then scanned
+    uint32_t* bidx;       // branching index of representative boundaries
+    uint32_t* cnt_hi;     // branching nodes whose key range ends at leaf i
+    uint32_t* suffix;     // S[i] = nodes with range end >= i (M+1 entries)
+    // per branching node (capacity N)
+    uint8_t* bdepth;
+    uint32_t* blo;
+    uint32_t* bhi;
+    int32_t* child;       // 8 per branching node, unified ids, -1 absent
+    // per unified node (capacity 2N)
+    uint32_t* parent;     // unified parent id (0xffffffff for the root)
+    uint32_t* windex;     // walk index
+    double* raw;          // 4 per node: mass, sum m*x, sum m*y, sum m*z
+    WalkNode* walk;       // capacity 2N
+    uint32_t* scan_tmp;
+    uint32_t* sort_tmp;
+    StepScalars* sc;
+};
+
+// kick (optional) + drift + box; writes sc->bb/span/side2.
+void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
+                           StepScalars* sc, double* partials, hipStream_t s);
+// Morton keys (src/morton_keys.cpp:65-98) + identity payload.
+void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key, uint32_t* perm,
+                 hipStream_t s);
+// Gather every body array through perm into dst; first-of-key flags.
+void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
+                    uint32_t* flag, size_t n, hipStream_t s);
+// Tree from sorted keys / bodies (needs flags already scanned in tb.flag).
+void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s);
+
+struct WalkParams {
+    double theta2, G, soft2, half_dt;
+    int kick;   // apply v += half_dt * a after the walk
+};
+void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const WalkParams& p,
+                       int64_t* cnt_accept, int64_t* cnt_open, int64_t* cnt_test, hipStream_t s);
+
+// FP64 direct sum for m sample bodies (Neumaier-compensated).
+void launch_direct_sum(const Bodies& b, size_t n, const int64_t* idx, size_t m, double G, double soft2,
+                       double* acc3, hipStream_t s);
+
+}  // namespace bh

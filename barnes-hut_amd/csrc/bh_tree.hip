@@ -1,0 +1,475 @@
+// bh_tree.hip -- kick/drift/box, Morton keys, body gather and the
+// bottom-up linear octree with its FP64 centre-of-mass reduction.
+//
+// Everything here is HBM-bound integer / FP64 work over the sorted bodies.
+// Each kernel reads its sizes (M leaves, B branching nodes) from device
+// memory, so a whole step runs without a host round trip.
+//
+// Tree.  The reference's map holds every (prefix, depth) cell that contains a
+// body, down to depth 21 (src/linear_octree.cpp:49-91).  Cells with a single
+// child form unary chains whose members all carry the same (mass, com) bits:
+// a new parent starts at 0.0 and adds its one child, and the sums never hold
+// -0.0.  The device tree keeps one node per chain: the distinct keys u[0..M)
+// are the leaves, and a branching node exists for every cell with two or more
+// children.  With delta_j the number of leading base-8 digits shared by u[j]
+// and u[j+1], the node whose children are split at boundary j is the cell of
+// depth delta_j around u[j]; its key range [lo, hi] and the last key of its
+// first child come from binary searches over u, and the first boundary of the
+// cell (the one inside its first child) represents it.
+//
+// Centre of mass (src/linear_octree.cpp:28-104): a leaf sums m and x*m over
+// its bodies in sorted order starting from 0.0; a branching node sums the raw
+// moments of its existing children in octant order 0..7 starting from 0.0;
+// com = raw / m for m > 0.  The one-rank full-tree exchange then rebuilds the
+// node as (0*0 + m*com) / (0 + m) (src/linear_octree.cpp:178-204).
+//
+// Walk order (src/traversal.cpp:59-98 pops children from octant 7 down to 0):
+// pre-order with children in descending key order.  Sorting nodes by (key
+// range end descending, tree depth ascending) gives exactly that order, so
+// walk index = (#nodes whose range ends after this one) + (#ancestors that end
+// at the same key), and the node after a subtree is #nodes whose range ends at
+// or after the subtree's first key.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+namespace {
+
+constexpr int kNT = 256;
+constexpr uint32_t kNone = 0xffffffffu;
+
+__device__ inline uint64_t prefix_at(uint64_t code, int depth)
+{
+    return depth == 0 ? 0ull : (code & (~0ull << (63 - 3 * depth)));
+}
+
+__device__ inline uint64_t stride_at(int depth) { return 1ull << (63 - 3 * depth); }
+
+// first index in u[0..m) with u[idx] >= x
+__device__ inline uint32_t lower_bound_u64(const uint64_t* __restrict__ u, uint32_t m, uint64_t x)
+{
+    uint32_t lo = 0, len = m;
+    while (len > 0) {
+        uint32_t half = len >> 1;
+        if (u[lo + half] < x) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// Number of leading base-8 digits two distinct 63-bit keys share (0..20).
+__device__ inline int split_depth(uint64_t a, uint64_t b)
+{
+    return (__clzll(static_cast<long long>(a ^ b)) - 1) / 3;
+}
+
+__device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
+
+// ------------------------------------------------------ kick/drift/box
+__global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int kick, double half_dt,
+                                                        int drift, double dt, StepScalars* sc,
+                                                        double* __restrict__ partials)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY};
+    double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const size_t stride = static_cast<size_t>(gridDim.x) * kNT;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x; i < n; i += stride) {
+        double v[3] = {b.vx[i], b.vy[i], b.vz[i]};
+        if (kick) {
+            // src/main.cpp:196-198: v += 0.5 * dt * a
+            v[0] = v[0] + half_dt * b.ax[i];
+            v[1] = v[1] + half_dt * b.ay[i];
+            v[2] = v[2] + half_dt * b.az[i];
+            b.vx[i] = v[0];
+            b.vy[i] = v[1];
+            b.vz[i] = v[2];
+        }
+        double x[3] = {b.px[i], b.py[i], b.pz[i]};
+        if (drift) {
+            // src/main.cpp:205-207: x += dt * v
+            x[0] = x[0] + dt * v[0];
+            x[1] = x[1] + dt * v[1];
+            x[2] = x[2] + dt * v[2];
+            b.px[i] = x[0];
+            b.py[i] = x[1];
+            b.pz[i] = x[2];
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = dmin(lo[a], x[a]);
+            hi[a] = dmax(hi[a], x[a]);
+        }
+    }
+    // block reduction
+    __shared__ double red[6][kNT / 64];
+    __shared__ bool last;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[a][w] = lo[a];
+            red[3 + a][w] = hi[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; ++a) {
+            double l = red[a][0], h = red[3 + a][0];
+            for (int k = 1; k < kNT / 64; ++k) {
+                l = dmin(l, red[a][k]);
+                h = dmax(h, red[3 + a][k]);
+            }
+            partials[6 * blockIdx.x + a] = l;
+            partials[6 * blockIdx.x + 3 + a] = h;
+        }
+        __threadfence();
+        const uint32_t prev = atomicAdd(&sc->block_counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        double bb[6];
+        for (int a = 0; a < 3; ++a) {
+            double l = INFINITY, h = -INFINITY;
+            for (unsigned k = 0; k < gridDim.x; ++k) {
+                l = dmin(l, __hip_atomic_load(&partials[6 * k + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                h = dmax(h, __hip_atomic_load(&partials[6 * k + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            }
+            bb[a] = l;
+            bb[3 + a] = h;
+        }
+        for (int k = 0; k < 6; ++k) sc->bb[k] = bb[k];
+        // src/morton_keys.cpp:71-73
+        for (int a = 0; a < 3; ++a) sc->span[a] = (bb[3 + a] == bb[a]) ? 1.0 : (bb[3 + a] - bb[a]);
+        // src/traversal.cpp:31-43
+        const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
+        double L = ex;
+        if (L < ey) L = ey;
+        if (L < ez) L = ez;
+        for (int d = 0; d <= kMaxDepth; ++d) {
+            const double side = L / static_cast<double>(1ull << d);
+            sc->side2[d] = side * side;
+        }
+        sc->error = 0;
+        sc->block_counter = 0;
+    }
+}
+
+// ------------------------------------------------------------- keys
+__device__ inline uint64_t spread3(uint32_t v)
+{
+    // bit b of v -> bit 3b: split into 21-bit groups by successive halving
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x001f00000000ffffull;
+    x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ inline uint32_t quantise(double v, double lo, double span)
+{
+    double t = (v - lo) / span;
+    if (t <= 0.0) t = 0.0;
+    else if (t >= 1.0) t = 0x1.fffffffffffffp-1;   // nextafter(1, 0)
+    return static_cast<uint32_t>(t * 2097152.0);
+}
+
+__global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, const StepScalars* __restrict__ sc,
+                                              uint64_t* __restrict__ key, uint32_t* __restrict__ perm)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t qx = quantise(b.px[i], sc->bb[0], sc->span[0]);
+    const uint32_t qy = quantise(b.py[i], sc->bb[1], sc->span[1]);
+    const uint32_t qz = quantise(b.pz[i], sc->bb[2], sc->span[2]);
+    key[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+    perm[i] = static_cast<uint32_t>(i);
+}
+
+// ---------------------------------------------------------- permute
+__global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, const uint32_t* __restrict__ perm,
+                                                 const uint64_t* __restrict__ key, uint32_t* __restrict__ flag,
+                                                 size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = perm[i];
+    dst.px[i] = src.px[j];
+    dst.py[i] = src.py[j];
+    dst.pz[i] = src.pz[j];
+    dst.vx[i] = src.vx[j];
+    dst.vy[i] = src.vy[j];
+    dst.vz[i] = src.vz[j];
+    dst.m[i] = src.m[j];
+    dst.id[i] = src.id[j];
+    flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+
+// ------------------------------------------------------------- tree
+// leaves: distinct keys, first body of each leaf, leaf id of each body
+__global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key, const uint32_t* __restrict__ flag,
+                                                const uint32_t* __restrict__ excl, size_t n, TreeBuffers tb)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t f = flag[i];
+    const uint32_t j = excl[i] + f - 1;
+    tb.leaf_of[i] = j;
+    if (f) {
+        tb.ukey[j] = key[i];
+        tb.leaf_first[j] = static_cast<uint32_t>(i);
+    }
+    if (i == 0) tb.leaf_first[tb.sc->n_leaves] = static_cast<uint32_t>(n);
+}
+
+// leaf raw moments; boundary depths and representatives
+__global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n, TreeBuffers tb)
+{
+    const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t M = tb.sc->n_leaves;
+    tb.cnt_hi[j] = 0;
+    if (j < M) {
+        // src/linear_octree.cpp:28-38: n.mass += m; n.com += x * m, from 0.0
+        double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        const uint32_t e = tb.leaf_first[j + 1];
+        for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
+            const double mi = b.m[i];
+            sm += mi;
+            sx += b.px[i] * mi;
+            sy += b.py[i] * mi;
+            sz += b.pz[i] * mi;
+        }
+        double* r = tb.raw + 4 * j;
+        r[0] = sm;
+        r[1] = sx;
+        r[2] = sy;
+        r[3] = sz;
+    }
+    uint32_t isrep = 0;
+    if (M > 1 && j < M - 1) {
+        const uint64_t a = tb.ukey[j];
+        const int d = split_depth(a, tb.ukey[j + 1]);
+        tb.delta[j] = static_cast<uint8_t>(d);
+        const uint32_t lo = lower_bound_u64(tb.ukey, M, prefix_at(a, d));
+        const uint64_t first_child = prefix_at(tb.ukey[lo], d + 1);
+        const uint32_t rep = lower_bound_u64(tb.ukey, M, first_child + stride_at(d + 1)) - 1;
+        tb.rep[j] = rep;
+        isrep = (rep == j) ? 1u : 0u;
+    }
+    tb.is_rep[j] = isrep;
+}
+
+// branching node records; boundary -> branching index
+__global__ __launch_bounds__(kNT) void k_branch_init(size_t n, TreeBuffers tb)
+{
+    const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t M = tb.sc->n_leaves;
+    if (!(M > 1 && j < M - 1)) return;
+    const uint32_t rep = tb.rep[j];
+    const bool is_rep = (rep == j);
+    tb.rep[j] = tb.bidx[rep];   // from here on: branching index of j's node
+    if (!is_rep) return;
+    const uint32_t bi = tb.bidx[j];
+    const int d = tb.delta[j];
+    const uint64_t p = prefix_at(tb.ukey[j], d);
+    tb.bdepth[bi] = static_cast<uint8_t>(d);
+    tb.blo[bi] = lower_bound_u64(tb.ukey, M, p);
+    tb.bhi[bi] = lower_bound_u64(tb.ukey, M, p + stride_at(d)) - 1;
+    int32_t* c = tb.child + 8 * static_cast<size_t>(bi);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = -1;
+}
+
+// parent links and octant slots of every node
+__global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (t >= static_cast<size_t>(M) + B) return;
+    uint32_t lo, hi;
+    if (t < M) {
+        lo = hi = static_cast<uint32_t>(t);
+    } else {
+        lo = tb.blo[t - M];
+        hi = tb.bhi[t - M];
+        atomicAdd(&tb.cnt_hi[hi], 1u);
+    }
+    const int dl = lo > 0 ? tb.delta[lo - 1] : -1;
+    const int dr = hi + 1 < M ? tb.delta[hi] : -1;
+    if (dl < 0 && dr < 0) {
+        tb.parent[t] = kNone;
+        tb.sc->root = static_cast<uint32_t>(t);
+        return;
+    }
+    const uint32_t bnd = (dl > dr) ? lo - 1 : hi;
+    const int pd = dl > dr ? dl : dr;
+    const uint32_t pb = tb.rep[bnd];
+    tb.parent[t] = M + pb;
+    const int oct = static_cast<int>((tb.ukey[lo] >> (63 - 3 * (pd + 1))) & 7u);
+    tb.child[8 * static_cast<size_t>(pb) + oct] = static_cast<int32_t>(t);
+}
+
+// raw moments of the branching nodes of one depth (src/linear_octree.cpp:69-87)
+__global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb, int depth)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (bi >= B || tb.bdepth[bi] != depth) return;
+    double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    const int32_t* c = tb.child + 8 * bi;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int32_t ck = c[k];
+        if (ck < 0) continue;
+        const double* r = tb.raw + 4 * static_cast<size_t>(ck);
+        sm += r[0];
+        sx += r[1];
+        sy += r[2];
+        sz += r[3];
+    }
+    double* r = tb.raw + 4 * (static_cast<size_t>(M) + bi);
+    r[0] = sm;
+    r[1] = sx;
+    r[2] = sy;
+    r[3] = sz;
+}
+
+// reversed per-leaf node counts for the suffix scan
+__global__ __launch_bounds__(kNT) void k_suffix_input(size_t n, TreeBuffers tb, uint32_t* __restrict__ out)
+{
+    const size_t k = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t M = tb.sc->n_leaves;
+    out[k] = k < M ? 1u + tb.cnt_hi[M - 1 - k] : 0u;
+}
+
+// S(i): number of nodes whose key range ends at or after leaf i
+__device__ inline uint32_t suffix_count(const TreeBuffers& tb, uint32_t M, uint32_t i)
+{
+    return i == 0 ? tb.sc->n_nodes : tb.suffix[M - i];
+}
+
+__device__ inline double finish_com(double raw, double m)
+{
+    // src/linear_octree.cpp:98-103, then :195-201 with m0 = com0 = 0
+    const double c = (m > 0) ? raw / m : raw;
+    const double z = 0.0;
+    return (z * z + m * c) / (z + m);
+}
+
+__global__ __launch_bounds__(kNT) void k_walk_records(size_t n, TreeBuffers tb)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (t >= static_cast<size_t>(M) + B) return;
+    const bool leaf = t < M;
+    uint32_t w, skip, lo, test_depth;
+    if (leaf) {
+        lo = static_cast<uint32_t>(t);
+        w = suffix_count(tb, M, lo) - 1;
+        skip = w + 1;
+        test_depth = kMaxDepth;
+    } else {
+        const uint32_t bi = static_cast<uint32_t>(t - M);
+        lo = tb.blo[bi];
+        const uint32_t hi = tb.bhi[bi];
+        uint32_t r = 0;
+        for (uint32_t p = tb.parent[t]; p != kNone && p >= M && tb.bhi[p - M] == hi; p = tb.parent[p]) ++r;
+        w = suffix_count(tb, M, hi + 1) + r;
+        skip = suffix_count(tb, M, lo);
+        test_depth = tb.bdepth[bi];
+    }
+    const uint32_t T = M + B;
+    if (w >= T || skip > T || skip <= w) {
+        atomicOr(&tb.sc->error, kErrWalkIndex);
+        return;
+    }
+    const uint32_t par = tb.parent[t];
+    const uint32_t top = (par == kNone) ? 0u : static_cast<uint32_t>(tb.bdepth[par - M]) + 1u;
+    tb.windex[t] = w;
+    const double* r = tb.raw + 4 * t;
+    const double m = r[0];
+    WalkNode nd;
+    if (0.0 + m == 0) {
+        nd.x = nd.y = nd.z = 0.0;
+        nd.m = 0.0;
+    } else {
+        nd.x = finish_com(r[1], m);
+        nd.y = finish_com(r[2], m);
+        nd.z = finish_com(r[3], m);
+        nd.m = 0.0 + m;
+    }
+    nd.side2 = tb.sc->side2[leaf ? kMaxDepth - 1 : test_depth];
+    nd.skip = skip;
+    uint32_t meta = test_depth | (top << kMetaTopShift);
+    if (leaf) {
+        meta |= kMetaLeaf;
+        if (top <= kMaxDepth - 1) meta |= kMetaChain;
+    }
+    nd.meta = meta;
+    tb.walk[w] = nd;
+    if (leaf) {
+        const uint32_t e = tb.leaf_first[t + 1];
+        for (uint32_t i = tb.leaf_first[t]; i < e; ++i) tb.self_w[i] = w;
+    }
+}
+
+inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kNT > 0 ? (n + kNT - 1) / kNT : 1); }
+
+}  // namespace
+
+void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
+                           StepScalars* sc, double* partials, hipStream_t s)
+{
+    unsigned g = nblk(n);
+    if (g > 1024) g = 1024;
+    k_kick_drift_box<<<g, kNT, 0, s>>>(b, n, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, sc, partials);
+}
+
+void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key, uint32_t* perm,
+                 hipStream_t s)
+{
+    k_keys<<<nblk(n), kNT, 0, s>>>(b, n, sc, key, perm);
+}
+
+void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
+                    uint32_t* flag, size_t n, hipStream_t s)
+{
+    k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, perm, key, flag, n);
+}
+
+void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s)
+{
+    // leaf index of every body: exclusive scan of first-of-key flags (M total)
+    scan_exclusive_u32(tb.flag, tb.leaf_of, &tb.sc->n_leaves, n, tb.scan_tmp, s);
+    k_leaves<<<nblk(n), kNT, 0, s>>>(tb.key, tb.flag, tb.leaf_of, n, tb);
+    k_leaf_moments_bounds<<<nblk(n), kNT, 0, s>>>(b, n, tb);
+    scan_exclusive_u32(tb.is_rep, tb.bidx, &tb.sc->n_branch, n, tb.scan_tmp, s);
+    k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
+    k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
+    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(n, tb, d);
+    k_suffix_input<<<nblk(n), kNT, 0, s>>>(n, tb, tb.is_rep);
+    scan_exclusive_u32(tb.is_rep, tb.suffix, &tb.sc->n_nodes, n, tb.scan_tmp, s);
+    k_walk_records<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
+}
+
+}  // namespace bh

@@ -1,0 +1,175 @@
+/*
+ * bh.h -- C ABI of the MI355X Barnes-Hut engine (libbh.so).
+ *
+ * The engine replaces the single-rank `--fc tree` force path of
+ * YonatanGM/Barnes-Hut: bounding box -> Morton keys -> key sort -> bottom-up
+ * linear octree with monopole centre of mass -> theta-criterion walk, inside
+ * the drift-first kick-drift-kick loop (src/main.cpp:191-281).  All entry
+ * points return BH_OK (0) or a negative BH_ERR_* code; no exception crosses
+ * the ABI and bh_last_error() describes the last failure of a context.
+ * Host buffers are caller-owned and sized by the caller; device memory is
+ * owned by the context.  One host thread per context; every call is
+ * synchronous on return.
+ */
+#ifndef BH_H
+#define BH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+#define BH_OK 0
+#define BH_ERR_ARG -1         /* bad argument */
+#define BH_ERR_NOMEM -2       /* host or device allocation failed */
+#define BH_ERR_HIP -3         /* a HIP runtime call failed */
+#define BH_ERR_STATE -4       /* call made in the wrong state (e.g. no bodies) */
+#define BH_ERR_IO -5          /* file could not be opened / written */
+#define BH_ERR_UNSUPPORTED -6 /* policy/form not provided by this engine */
+#define BH_ERR_OVERFLOW -7    /* caller buffer too small */
+
+/* Gravitational constant in AU^3 kg^-1 day^-2 (src/main.cpp:174). */
+#define BH_G 1.48812e-34
+/* Tree depth of a leaf: 21 bits per axis (src/linear_octree.cpp:10). */
+#define BH_MAX_DEPTH 21
+
+/* Body vector, layout-identical to `struct Vec3 {double x,y,z;}`
+ * (include/body.h:4-6) so std::vector<Vec3>::data() can be passed as is. */
+typedef struct bh_vec3 {
+    double x, y, z;
+} bh_vec3;
+
+/* Force-exchange policy (include/policy.h:10-15).  One GPU holds every body,
+ * so only the full-tree policy exists; LET values are rejected with
+ * BH_ERR_UNSUPPORTED. */
+typedef enum bh_fc_policy { BH_FC_TREE = 0, BH_FC_LET = 1, BH_FC_LET_DIRECT = 2 } bh_fc_policy;
+
+/* Walk form.
+ *  PLAIN   FP64 walk, same decisions and summation order as
+ *          src/traversal.cpp:13-103 on the same tree (bit-identical to the
+ *          reference built without FMA contraction).
+ *  MIXED   FP64 state, drift, box, keys, COM and the central-mass term; the
+ *          bulk of the walk in FP32 on leaf-relative offsets with compensated
+ *          accumulation.
+ *  LEARNED MIXED walk whose leaves are ordered and split into balanced
+ *          persistent-kernel chunks by a cost model; forces are bit-identical
+ *          to MIXED. */
+typedef enum bh_form { BH_FORM_PLAIN = 0, BH_FORM_MIXED = 1, BH_FORM_LEARNED = 2 } bh_form;
+
+typedef struct bh_config {
+    int device;       /* HIP device ordinal */
+    int fc_policy;    /* bh_fc_policy; BH_FC_TREE */
+    int form;         /* bh_form used by bh_step */
+    double theta;     /* opening angle (--theta, default 1.05) */
+    double G;         /* gravitational constant (BH_G) */
+    double soft2;     /* softening eps^2 added to r^2 (0.0 in the reference) */
+} bh_config;
+
+typedef struct bh_ctx bh_ctx;
+
+/* Fill *cfg with the reference defaults: device 0, full tree, PLAIN,
+ * theta 1.05 (src/main.cpp:61), G = BH_G, soft2 = 0. */
+void bh_config_default(bh_config *cfg);
+
+/* Context lifetime.  bh_create initialises the HIP device; it fails with
+ * BH_ERR_HIP when no device is usable and with BH_ERR_UNSUPPORTED for a LET
+ * policy. */
+int bh_create(bh_ctx **out, const bh_config *cfg);
+void bh_destroy(bh_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none).  Valid until the
+ * next call on ctx.  With ctx == NULL: the last bh_create failure. */
+const char *bh_last_error(const bh_ctx *ctx);
+
+/* Upload n bodies (replaces the reference's rank-0 read + scatter,
+ * src/main.cpp:117-171).  ids may be NULL (then ids = 0..n-1 in row order,
+ * as src/io.cpp:73).  Resets time to 0 and the accelerations to zero
+ * (src/main.cpp:175). */
+int bh_set_bodies(bh_ctx *ctx, int64_t n, const bh_vec3 *pos, const bh_vec3 *vel,
+                  const double *mass, const uint64_t *ids);
+/* Number of bodies held by ctx. */
+int64_t bh_num_bodies(const bh_ctx *ctx);
+/* Download the bodies in their current (last Morton-sorted) order.  Any
+ * pointer may be NULL. */
+int bh_get_bodies(bh_ctx *ctx, bh_vec3 *pos, bh_vec3 *vel, double *mass, uint64_t *ids, bh_vec3 *acc);
+
+/* The drop-in for one `--fc tree` force evaluation (src/main.cpp:212-263):
+ * bounding box, Morton keys, stable key sort that permutes the bodies
+ * persistently (include/utility.h:23-52), linear octree with the one-rank
+ * exchange round trip (src/linear_octree.cpp:13-107,169-204), and the walk
+ * (src/traversal.cpp:13-103).  acc_out (may be NULL) receives the
+ * accelerations in the new body order; they are also kept on the device for
+ * the next kick. */
+int bh_compute_accelerations(bh_ctx *ctx, int form, double theta, double G, double soft2,
+                             bh_vec3 *acc_out);
+
+/* nsteps iterations of the drift-first KDK loop (src/main.cpp:191-281):
+ * half kick (skipped while t == 0), drift, force evaluation with ctx's form,
+ * theta, G and soft2, half kick; t += dt.  Everything stays on the device. */
+int bh_step(bh_ctx *ctx, int64_t nsteps, double dt);
+/* Simulation time t accumulated by bh_step. */
+double bh_time(const bh_ctx *ctx);
+/* Number of iterations `for (t = 0; t < t_end; t += dt)` performs
+ * (src/main.cpp:191): e.g. dt = 1h, t_end = 100h gives 101. */
+int64_t bh_count_steps(double dt, double t_end);
+/* Parse "1h" / "2.5d" / "1y" into days (include/parse_time.h:6-16). */
+int bh_parse_time(const char *text, double *days);
+
+/* ---------------------------------------------- exports for tests --- */
+/* Sorted Morton keys and box of the last force evaluation. */
+int bh_export_keys(bh_ctx *ctx, uint64_t *keys);
+int bh_export_bbox(bh_ctx *ctx, double *bb6);
+/* Node counts of the last tree: full-chain node count of the reference's
+ * map (every (prefix, depth) that holds a body) and compressed node count of
+ * the device tree. */
+int bh_tree_size(bh_ctx *ctx, int64_t *n_full, int64_t *n_compressed);
+/* The reference's node set of the last tree sorted by (depth, prefix) with
+ * the (mass, com) the walk reads; arrays sized n_full (com: 3*n_full). */
+int bh_export_topology(bh_ctx *ctx, uint64_t *prefix, uint8_t *depth, double *mass, double *com);
+/* Per-body counts of the last PLAIN walk: accepted nodes and opened nodes of
+ * the full-chain tree (what src/traversal.cpp:63-98 would count), and the
+ * number of compressed nodes the body tested (accepted + opened + its own
+ * leaf).  Enabled by bh_set_counting(ctx, 1); counting costs walk time.  Any
+ * output pointer may be NULL. */
+int bh_set_counting(bh_ctx *ctx, int on);
+int bh_get_counters(bh_ctx *ctx, int64_t *accepted, int64_t *opened, int64_t *tested);
+/* Device time in ms of the phases of the last force evaluation:
+ * [0] kick+drift+box+keys, [1] sort+permute, [2] tree build, [3] walk(+kick),
+ * [4] total. */
+int bh_get_phase_ms(bh_ctx *ctx, double *ms5);
+/* Mean device time per step of the same phases over the steps of the last
+ * bh_step call (at most the last 1024 steps); *nsteps gets the count. */
+int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
+
+/* FP64 direct sum on the device for m sample bodies (indices into the
+ * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 with
+ * Neumaier-compensated sums, r^2 = |d|^2 + soft2.  The reference for force
+ * errors; acc_out is m x 3. */
+int bh_direct_sum(bh_ctx *ctx, int64_t m, const int64_t *idx, double G, double soft2, bh_vec3 *acc_out);
+
+/* ------------------------------------------------ synthetic inputs --- */
+#define BH_GEN_BELT 0   /* central mass + 8 planets + main belt */
+#define BH_GEN_KUIPER 1 /* as BELT with 30% of the asteroids in a Kuiper belt */
+#define BH_GEN_TROJAN 2 /* 60% belt, 2 x 15% Jupiter Trojans, 10% Hildas */
+#define BH_GEN_CENTRAL_MASS 1.989e30
+
+/* n >= 9 bodies: [0] central mass at rest at the origin, [1..8] planets,
+ * then asteroids.  Deterministic in (kind, seed, n). */
+int bh_generate(int kind, uint64_t seed, int64_t n, bh_vec3 *pos, bh_vec3 *vel, double *mass);
+/* count system sizes, log-uniform in [nmin, nmax]. */
+int bh_generate_ensemble_sizes(uint64_t seed, int64_t count, int64_t nmin, int64_t nmax, int64_t *sizes);
+/* State-vector CSV in the reference format (src/io.cpp:16-81). */
+int bh_write_csv(const char *path, int64_t n, const bh_vec3 *pos, const bh_vec3 *vel, const double *mass);
+/* Reads at most body_limit bodies (<= 0: all).  With pos/vel/mass NULL only
+ * counts (*n_out); otherwise fills up to capacity rows and returns
+ * BH_ERR_OVERFLOW when more rows were found.  ids (may be NULL) = row order
+ * of the parsed rows; malformed rows are skipped with a warning. */
+int bh_read_csv(const char *path, int64_t body_limit, int64_t capacity, int64_t *n_out, bh_vec3 *pos,
+                bh_vec3 *vel, double *mass, uint64_t *ids);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BH_H */
