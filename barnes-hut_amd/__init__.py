@@ -30,6 +30,7 @@ OK = 0
 ERR_ARG, ERR_NOMEM, ERR_HIP, ERR_STATE, ERR_IO, ERR_UNSUPPORTED, ERR_OVERFLOW = -1, -2, -3, -4, -5, -6, -7
 FC_TREE, FC_LET, FC_LET_DIRECT = 0, 1, 2
 FORM_PLAIN, FORM_MIXED, FORM_LEARNED = 0, 1, 2
+SCHED_KEY_ORDER, SCHED_PREV_COST, SCHED_MODEL = 0, 1, 2
 GEN_BELT, GEN_KUIPER, GEN_TROJAN = 0, 1, 2
 G = 1.48812e-34
 MAX_DEPTH = 21
@@ -74,6 +75,8 @@ SIGNATURES = [
     ("bh_get_phase_ms", C.c_int, [_P, _P]),
     ("bh_get_step_phase_ms", C.c_int, [_P, _P, C.POINTER(_I64)]),
     ("bh_direct_sum", C.c_int, [_P, _I64, _P, _D, _D, _P]),
+    ("bh_get_groups", C.c_int, [_P, C.POINTER(_I64), _I64, _P, _P, _P]),
+    ("bh_set_schedule", C.c_int, [_P, C.c_int]),
     ("bh_generate", C.c_int, [C.c_int, C.c_uint64, _I64, _P, _P, _P]),
     ("bh_generate_ensemble_sizes", C.c_int, [C.c_uint64, _I64, _I64, _I64, _P]),
     ("bh_write_csv", C.c_int, [C.c_char_p, _I64, _P, _P, _P]),
@@ -279,6 +282,21 @@ class Engine:
         k = _I64(0)
         self._ck(self.lib.bh_get_step_phase_ms(self.h, _ptr(out), C.byref(k)))
         return out, k.value
+
+    def set_schedule(self, mode: int):
+        self._ck(self.lib.bh_set_schedule(self.h, mode))
+
+    def groups(self):
+        """Walk groups of the last mixed evaluation: first_body (G+1),
+        interactions (G), visited (G)."""
+        k = _I64(0)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
+        g = k.value
+        first = np.zeros(g + 1, np.uint32)
+        inter = np.zeros(g, np.uint32)
+        visit = np.zeros(g, np.uint32)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), g, _ptr(first), _ptr(inter), _ptr(visit)))
+        return first, inter, visit
 
     def direct_sum(self, idx, G_: float = G, soft2: float = 0.0) -> np.ndarray:
         """FP64 direct-sum accelerations (device) for bodies idx of the current order."""

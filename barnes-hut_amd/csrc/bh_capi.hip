@@ -21,6 +21,7 @@
 #include "bh_internal.hpp"
 
 using bh::Bodies;
+using bh::MixedBuffers;
 using bh::StepScalars;
 using bh::TreeBuffers;
 using bh::WalkNode;
@@ -33,6 +34,11 @@ struct bh_ctx {
     double t = 0.0;
     Bodies cur{}, alt{};
     TreeBuffers tb{};
+    MixedBuffers mb{};
+    int n_heavy = 0;
+    unsigned mixed_blocks = 0;
+    int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
+    bool model_ready = false;         // the LEARNED cost model is trained
     double* partials = nullptr;
     int64_t* cnt_a = nullptr;
     int64_t* cnt_o = nullptr;
@@ -40,6 +46,7 @@ struct bh_ctx {
     bool counting = false;
     bool have_tree = false;
     bool have_counts = false;
+    bool have_groups = false;
     hipEvent_t ev[5] = {};
     // per-step phase events of bh_step: ring of kRing sets of 5
     std::vector<hipEvent_t> ring;
@@ -78,6 +85,7 @@ void free_buffers(bh_ctx* c)
     c->cur = Bodies{};
     c->alt = Bodies{};
     c->tb = TreeBuffers{};
+    c->mb = MixedBuffers{};
     c->partials = nullptr;
     c->cnt_a = c->cnt_o = c->cnt_t = nullptr;
     c->cap = 0;
@@ -108,7 +116,7 @@ int reserve(bh_ctx* c, size_t n)
     for (Bodies* b : {&c->cur, &c->alt}) {
         A(b->px, N); A(b->py, N); A(b->pz, N);
         A(b->vx, N); A(b->vy, N); A(b->vz, N);
-        A(b->m, N); A(b->id, N);
+        A(b->m, N); A(b->id, N); A(b->hrank, N); A(b->pcost, N);
     }
     A(c->cur.ax, N); A(c->cur.ay, N); A(c->cur.az, N);
     c->alt.ax = c->cur.ax; c->alt.ay = c->cur.ay; c->alt.az = c->cur.az;
@@ -118,7 +126,13 @@ int reserve(bh_ctx* c, size_t n)
     A(t.ukey, N); A(t.leaf_first, N + 1); A(t.delta, N); A(t.rep, N); A(t.is_rep, N);
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
-    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.walk, 2 * N);
+    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.walk, 2 * N);
+    A(t.heavy_slot, 64); A(t.gflag, N);
+    MixedBuffers& m = c->mb;
+    A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
+    A(m.g_inter, N); A(m.g_visit, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
+    A(m.sched_hist, 64); A(m.sched_cursor, 64);
+    HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
     A(t.sc, 1);
     A(c->partials, 6 * 1024);
@@ -130,13 +144,17 @@ int reserve(bh_ctx* c, size_t n)
 
 // One force evaluation with optional leading kick + drift and trailing kick.
 constexpr int kRing = 512;
+constexpr int kHeavy = 16;
 
 int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool kick_before, bool drift,
               double dt, bool kick_after, const hipEvent_t* ev)
 {
-    if (form != BH_FORM_PLAIN)
-        return fail(c, BH_ERR_UNSUPPORTED, "form %d is not available in this build", form);
+    if (form != BH_FORM_PLAIN && form != BH_FORM_MIXED && form != BH_FORM_LEARNED)
+        return fail(c, BH_ERR_ARG, "unknown form %d", form);
     if (c->n == 0) return BH_OK;
+    const bool mixed = form != BH_FORM_PLAIN;
+    if (mixed && c->n >= (size_t{1} << 26))
+        return fail(c, BH_ERR_ARG, "the mixed forms take at most 2^26 - 1 bodies");
     hipStream_t s = c->stream;
     const double half_dt = 0.5 * dt;
     HIP_TRY(c, hipEventRecord(ev[0], s));
@@ -144,10 +162,34 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     bh::launch_keys(c->cur, c->n, c->tb.sc, c->tb.key, c->tb.perm, s);
     HIP_TRY(c, hipEventRecord(ev[1], s));
     bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63, c->tb.sort_tmp, s);
-    bh::launch_permute(c->cur, c->alt, c->tb.perm, c->tb.key, c->tb.flag, c->n, s);
+    bh::launch_permute(c->cur, c->alt, c->tb.perm, c->tb.key, c->tb.flag, c->tb.heavy_slot, c->n, s);
     std::swap(c->cur, c->alt);
     HIP_TRY(c, hipEventRecord(ev[2], s));
-    bh::launch_build_tree(c->cur, c->n, c->tb, s);
+    TreeBuffers tb = c->tb;
+    if (!mixed) {
+        tb.lraw = nullptr;
+        tb.gflag = nullptr;
+    }
+    bh::launch_build_tree(c->cur, c->n, tb, s);
+    if (mixed) {
+        bh::launch_mixed_prepare(c->n, tb, c->mb, G, s);
+        int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;
+        if (sched == BH_SCHED_MODEL && !c->model_ready) sched = BH_SCHED_PREV_COST;
+        if (sched != BH_SCHED_KEY_ORDER)
+            bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, s);
+        HIP_TRY(c, hipEventRecord(ev[3], s));
+        bh::MixedParams mp{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0, c->n_heavy,
+                           sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4};
+        bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
+        HIP_TRY(c, hipEventRecord(ev[4], s));
+        HIP_TRY(c, hipGetLastError());
+        c->last_ev = ev;
+        c->have_tree = false;   // the plain walk order was not built
+        c->have_counts = false;
+        c->have_groups = true;
+        return BH_OK;
+    }
+    bh::launch_walk_order(c->n, tb, s);
     HIP_TRY(c, hipEventRecord(ev[3], s));
     bh::WalkParams p{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0};
     const bool cnt = c->counting;
@@ -168,6 +210,16 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
 }
 
 int check_ctx(const bh_ctx* c) { return c ? BH_OK : BH_ERR_ARG; }
+
+// Error bits raised by device-side guards during the last evaluation.
+int check_device_errors(bh_ctx* c)
+{
+    uint32_t err = 0;
+    HIP_TRY(c, hipMemcpy(&err, &c->tb.sc->error, sizeof err, hipMemcpyDeviceToHost));
+    if (err & bh::kErrStack) return fail(c, BH_ERR_STATE, "mixed walk: traversal stack capacity exceeded");
+    if (err) return fail(c, BH_ERR_STATE, "device guard raised error bits 0x%x", err);
+    return BH_OK;
+}
 
 void to_soa(const bh_vec3* v, size_t n, std::vector<double>& x, std::vector<double>& y, std::vector<double>& z)
 {
@@ -267,6 +319,7 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
     if (rc) return rc;
     c->n = static_cast<size_t>(n);
     c->t = 0.0;
+    if (c->mixed_blocks == 0) c->mixed_blocks = bh::mixed_walk_blocks(c->cfg.device);
     c->have_tree = false;
     c->have_counts = false;
     if (n == 0) return BH_OK;
@@ -289,6 +342,23 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
         ids = idv.data();
     }
     HIP_TRY(c, hipMemcpyAsync(c->cur.id, ids, n * 8, hipMemcpyHostToDevice, s));
+    {
+        // heavy set of the mixed forms: the kHeavy most massive bodies (ties by
+        // row), summed directly in FP64; ranks follow row order
+        const int k = static_cast<int>(std::min<int64_t>(kHeavy, n));
+        std::vector<int64_t> idx(c->n);
+        for (size_t i = 0; i < c->n; ++i) idx[i] = static_cast<int64_t>(i);
+        std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](int64_t a, int64_t b2) {
+            return mass[a] > mass[b2] || (mass[a] == mass[b2] && a < b2);
+        });
+        std::sort(idx.begin(), idx.begin() + k);
+        std::vector<int8_t> hr(c->n, -1);
+        for (int r = 0; r < k; ++r) hr[idx[r]] = static_cast<int8_t>(r);
+        c->n_heavy = k;
+        HIP_TRY(c, hipMemcpyAsync(c->cur.hrank, hr.data(), n, hipMemcpyHostToDevice, s));
+        HIP_TRY(c, hipMemsetAsync(c->cur.pcost, 0, n * sizeof(uint32_t), s));
+        HIP_TRY(c, hipStreamSynchronize(s));
+    }
     HIP_TRY(c, hipMemsetAsync(c->cur.ax, 0, n * 8, s));
     HIP_TRY(c, hipMemsetAsync(c->cur.ay, 0, n * 8, s));
     HIP_TRY(c, hipMemsetAsync(c->cur.az, 0, n * 8, s));
@@ -320,6 +390,7 @@ int bh_compute_accelerations(bh_ctx* c, int form, double theta, double G, double
     int rc = run_force(c, form, theta, G, soft2, false, false, 0.0, false, c->ev);
     if (rc) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_device_errors(c))) return rc;
     return download3(c, acc_out, c->cur.ax, c->cur.ay, c->cur.az);
 }
 
@@ -341,7 +412,7 @@ int bh_step(bh_ctx* c, int64_t nsteps, double dt)
         c->ring_steps = k + 1;
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return BH_OK;
+    return check_device_errors(c);
 }
 
 double bh_time(const bh_ctx* c) { return c ? c->t : 0.0; }
@@ -545,6 +616,31 @@ int bh_get_step_phase_ms(bh_ctx* c, double* ms5, int64_t* nsteps)
     const int64_t cnt = c->ring_steps - k0;
     for (int q = 0; q < 5; ++q) ms5[q] = acc[q] / static_cast<double>(cnt);
     if (nsteps) *nsteps = cnt;
+    return BH_OK;
+}
+
+int bh_set_schedule(bh_ctx* c, int mode)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (mode < BH_SCHED_KEY_ORDER || mode > BH_SCHED_MODEL) return fail(c, BH_ERR_ARG, "unknown schedule %d", mode);
+    c->sched = mode;
+    return BH_OK;
+}
+
+int bh_get_groups(bh_ctx* c, int64_t* n_groups, int64_t capacity, uint32_t* first_body, uint32_t* interactions,
+                  uint32_t* visited)
+{
+    if (check_ctx(c) || !n_groups) return BH_ERR_ARG;
+    if (!c->have_groups) return fail(c, BH_ERR_STATE, "no mixed evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    *n_groups = sc.n_groups;
+    if (capacity < static_cast<int64_t>(sc.n_groups)) return BH_OK;
+    const size_t G = sc.n_groups;
+    if (first_body) HIP_TRY(c, hipMemcpy(first_body, c->mb.gfirst, (G + 1) * 4, hipMemcpyDeviceToHost));
+    if (interactions) HIP_TRY(c, hipMemcpy(interactions, c->mb.g_inter, G * 4, hipMemcpyDeviceToHost));
+    if (visited) HIP_TRY(c, hipMemcpy(visited, c->mb.g_visit, G * 4, hipMemcpyDeviceToHost));
     return BH_OK;
 }
 

@@ -51,9 +51,12 @@ struct StepScalars {
     uint64_t n_full;     // node count of the reference's full-chain map
     uint32_t block_counter;  // last-block-done counter, left at 0
     uint32_t error;          // kErr* bits raised by bounds guards
+    uint32_t n_groups;       // walk groups of the mixed forms
+    uint32_t group_counter;  // dynamic work queue of the mixed walk
 };
 
 constexpr uint32_t kErrWalkIndex = 1u;   // a walk index fell outside [0, M+B)
+constexpr uint32_t kErrStack = 2u;       // a mixed-walk stack would have overflowed
 
 // Scratch sizing helpers
 inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -77,6 +80,8 @@ void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, si
 struct Bodies {
     double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *m;
     uint64_t* id;
+    int8_t* hrank;   // rank in the heavy (direct FP64) set, -1 for light bodies
+    uint32_t* pcost; // list length the body's group evaluated in the last mixed walk
 };
 
 struct TreeBuffers {
@@ -106,6 +111,9 @@ struct TreeBuffers {
     uint32_t* parent;     // unified parent id (0xffffffff for the root)
     uint32_t* windex;     // walk index
     double* raw;          // 4 per node: mass, sum m*x, sum m*y, sum m*z
+    double* lraw;         // the same over light bodies only (mixed forms), or null
+    uint32_t* heavy_slot; // sorted position of each heavy body, by rank
+    uint32_t* gflag;      // walk-group start flags (mixed forms), cleared per step
     WalkNode* walk;       // capacity 2N
     uint32_t* scan_tmp;
     uint32_t* sort_tmp;
@@ -120,9 +128,12 @@ void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key
                  hipStream_t s);
 // Gather every body array through perm into dst; first-of-key flags.
 void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
-                    uint32_t* flag, size_t n, hipStream_t s);
-// Tree from sorted keys / bodies (needs flags already scanned in tb.flag).
+                    uint32_t* flag, uint32_t* heavy_slot, size_t n, hipStream_t s);
+// Tree topology and raw moments from sorted keys / bodies and first-of-key
+// flags (tb.flag); light moments too when tb.lraw is set.
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s);
+// Walk-order records of the plain walk (after launch_build_tree).
+void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s);
 
 struct WalkParams {
     double theta2, G, soft2, half_dt;
@@ -130,6 +141,57 @@ struct WalkParams {
 };
 void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const WalkParams& p,
                        int64_t* cnt_accept, int64_t* cnt_open, int64_t* cnt_test, hipStream_t s);
+
+// ------------------------------------------------------- mixed forms
+// Node of the mixed walk.  Children of a node occupy consecutive slots, so
+// opening a node pushes one (first, count) range.  Masses and centres are
+// those of the light bodies; the heavy bodies are summed directly in FP64.
+struct MNode {
+    double x, y, z;   // centre of mass of the light bodies (FP64)
+    double side2;     // (L / 2^d)^2 at the chain bottom d (branching nodes)
+    float gm;         // G * light mass
+    uint32_t first;   // first child slot (branching) or first body (leaf)
+    uint32_t info;    // child or body count | kMLeaf
+    uint32_t pad;
+};
+static_assert(sizeof(MNode) == 48, "MNode is 48 bytes");
+constexpr uint32_t kMLeaf = 1u << 31;
+constexpr int kGroupMax = 64;   // bodies per walk group (one wavefront)
+
+struct MixedBuffers {
+    MNode* node;        // 2N
+    uint32_t* slot;     // 2N: unified id -> slot
+    uint32_t* base;     // N: first child slot of each branching node
+    uint32_t* gid;      // N: group start flags scanned
+    uint32_t* gfirst;   // N + 1: first body of each group
+    uint8_t* gdepth;    // N: tree depth of the group's cells (at the start body)
+    uint32_t* g_inter;  // N: interactions evaluated by each group
+    uint32_t* g_visit;  // N: nodes visited by each group
+    uint32_t* order;    // N: group order of the scheduled walk
+    uint32_t* chunk;    // workers + 1: static chunk starts into order[]
+    float* gcost;       // N: estimated cost of each group
+    uint32_t* sched_hist;    // 64 cost buckets (left at zero)
+    uint32_t* sched_cursor;  // 64
+};
+
+struct MixedParams {
+    double theta2, G, soft2, half_dt;
+    int kick;
+    int n_heavy;
+    int sched;            // 0 dynamic in key order, 1 dynamic over order[], 2 static chunks
+    uint32_t n_workers;
+};
+
+// Walk groups and the children-contiguous node array (after the tree build).
+void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s);
+// Persistent mixed walk over the groups; returns the number of worker waves.
+void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
+                       const MixedParams& p, unsigned blocks, hipStream_t s);
+unsigned mixed_walk_blocks(int device);
+// Heaviest-first group order into mb.order (source 0: previous step's list
+// lengths; 1: mb.gcost from the cost model).
+void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
+                     hipStream_t s);
 
 // FP64 direct sum for m sample bodies (Neumaier-compensated).
 void launch_direct_sum(const Bodies& b, size_t n, const int64_t* idx, size_t m, double G, double soft2,

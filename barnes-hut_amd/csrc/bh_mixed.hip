@@ -1,0 +1,576 @@
+// bh_mixed.hip -- the precision form of the walk: FP32 on group-relative
+// offsets with compensated accumulation, FP64 for the heavy bodies.
+//
+// Walk groups.  Bodies are cut into groups of at most 64 Morton-consecutive
+// bodies that follow the tree: a cell with <= 64 bodies whose parent holds
+// more is a bucket; consecutive sibling buckets are packed while they fit in
+// 64; a leaf with more than 64 co-keyed bodies is split.  One wavefront walks
+// the tree once per group, one lane per body.
+//
+// Opening test per group: a node is accepted when side(d)^2 < theta^2 * D^2,
+// D the FP64 distance from the node's centre of mass to the group's bounding
+// box.  Every body of the group is at least D from the centre of mass, so the
+// group test implies the reference's per-body test (src/traversal.cpp:78) for
+// all of them.  The result of a group depends only on the group, never on
+// which worker walks it or when: every schedule gives identical bits.
+//
+// Traversal keeps a stack of node ranges in LDS (children of a node occupy
+// consecutive slots); each round pops up to 64 ranges, one per lane, and the
+// lanes test the nodes of their range in FP64.  Accepted nodes and the bodies
+// of reached leaves go into an LDS interaction list as FP32 (x - c, y - c,
+// z - c, G m) with c the group centre (FP64), so the offsets keep FP32
+// precision near the group.  A full list is evaluated by all 64 lanes:
+// FP32 FMA over blocks of 32 entries, each block sum added into a
+// Kahan-compensated FP32 accumulator.  A group's own bodies appear in the
+// list with offsets bit-identical to the lane's own, so r^2 = 0 marks (and
+// drops) the self term.
+//
+// Heavy bodies (the central mass and planets: the heaviest bodies, up to 16)
+// are not in the light moments; every body adds them by an FP64 direct sum in
+// rank order.  The sum of the two parts is FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kWaves = kNT / 64;
+constexpr int kListCap = 256;
+constexpr int kStackCap = 1024;
+// Up to 64 ranges may be popped while the stack holds at most this many
+// entries: one round pushes at most 8 ranges per popped range, and a
+// single-range round (pure depth first) grows the stack by at most 7 per level.
+constexpr int kBigPopLimit = kStackCap - 7 * 64 - 8 * 22;   // see the pop rule in k_walk_mixed
+constexpr uint32_t kNone = 0xffffffffu;
+
+// stack entry: first (27 bits) | count - 1 (3 bits, ranges hold <= 8) | body range (bit 31)
+__device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
+{
+    return first | ((count - 1u) << 27) | (body ? 0x80000000u : 0u);   // count in [1, 8]
+}
+
+__device__ inline uint32_t unit_count(const TreeBuffers& tb, uint32_t M, uint32_t t)
+{
+    if (t < M) return tb.leaf_first[t + 1] - tb.leaf_first[t];
+    const uint32_t bi = t - M;
+    return tb.leaf_first[tb.bhi[bi] + 1] - tb.leaf_first[tb.blo[bi]];
+}
+
+__device__ inline uint32_t unit_first_body(const TreeBuffers& tb, uint32_t M, uint32_t t)
+{
+    return t < M ? tb.leaf_first[t] : tb.leaf_first[tb.blo[t - M]];
+}
+
+// group starts (flags at the first body of every group)
+__global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, MixedBuffers mb)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (t >= static_cast<size_t>(M) + B) return;
+    const uint32_t u = static_cast<uint32_t>(t);
+    const uint32_t cnt = unit_count(tb, M, u);
+    const uint32_t depth = u < M ? kMaxDepth : tb.bdepth[u - M];
+    if (tb.parent[u] == kNone && cnt <= kGroupMax) {
+        const uint32_t f = unit_first_body(tb, M, u);
+        tb.gflag[f] = 1;
+        mb.gdepth[f] = static_cast<uint8_t>(depth);
+        return;
+    }
+    if (u < M) {
+        if (cnt > kGroupMax) {   // co-keyed bodies beyond one group
+            const uint32_t f = tb.leaf_first[u];
+            for (uint32_t k = 0; k < cnt; k += kGroupMax) {
+                tb.gflag[f + k] = 1;
+                mb.gdepth[f + k] = kMaxDepth;
+            }
+        }
+        return;
+    }
+    if (cnt <= kGroupMax) return;
+    const int32_t* c = tb.child + 8 * static_cast<size_t>(u - M);
+    uint32_t cur = 0;
+    bool open = false;
+    for (int k = 0; k < 8; ++k) {
+        const int32_t ck = c[k];
+        if (ck < 0) continue;
+        const uint32_t cc = unit_count(tb, M, static_cast<uint32_t>(ck));
+        if (cc > kGroupMax) {
+            open = false;
+            continue;
+        }
+        if (!open || cur + cc > kGroupMax) {
+            const uint32_t f = unit_first_body(tb, M, static_cast<uint32_t>(ck));
+            tb.gflag[f] = 1;
+            mb.gdepth[f] = static_cast<uint8_t>(depth + 1);
+            cur = 0;
+            open = true;
+        }
+        cur += cc;
+    }
+}
+
+__global__ __launch_bounds__(kNT) void k_group_first(size_t n, TreeBuffers tb, MixedBuffers mb)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= n) return;
+    if (tb.gflag[i]) mb.gfirst[mb.gid[i]] = static_cast<uint32_t>(i);
+    if (i == 0) mb.gfirst[tb.sc->n_groups] = static_cast<uint32_t>(n);
+}
+
+// number of children of every branching node (into base, scanned next)
+__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, uint32_t* __restrict__ out)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (bi >= n) return;
+    uint32_t c = 0;
+    if (bi < tb.sc->n_branch) {
+        const int32_t* ch = tb.child + 8 * bi;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
+    }
+    out[bi] = c;
+}
+
+// slots: root 0; children of branching node b at 1 + base[b] + rank
+__global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (bi == 0) mb.slot[tb.sc->root] = 0;
+    if (bi >= tb.sc->n_branch) return;
+    const int32_t* ch = tb.child + 8 * bi;
+    uint32_t k = 0;
+    const uint32_t b0 = 1u + mb.base[bi];
+#pragma unroll
+    for (int o = 0; o < 8; ++o)
+        if (ch[o] >= 0) mb.slot[ch[o]] = b0 + k++;
+}
+
+__global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, TreeBuffers tb, MixedBuffers mb, double G)
+{
+    const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (t >= static_cast<size_t>(M) + B) return;
+    const double* q = tb.lraw + 4 * t;
+    const double m = q[0];
+    MNode nd;
+    if (m != 0.0) {
+        nd.x = q[1] / m;
+        nd.y = q[2] / m;
+        nd.z = q[3] / m;
+    } else {
+        nd.x = nd.y = nd.z = 0.0;
+    }
+    nd.gm = static_cast<float>(G * m);
+    nd.pad = 0;
+    if (t < M) {
+        nd.side2 = 0.0;
+        nd.first = tb.leaf_first[t];
+        nd.info = (tb.leaf_first[t + 1] - tb.leaf_first[t]) | kMLeaf;
+    } else {
+        const uint32_t bi = static_cast<uint32_t>(t - M);
+        uint32_t c = 0;
+        const int32_t* ch = tb.child + 8 * static_cast<size_t>(bi);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
+        nd.side2 = tb.sc->side2[tb.bdepth[bi]];
+        nd.first = 1u + mb.base[bi];
+        nd.info = c;
+    }
+    mb.node[mb.slot[t]] = nd;
+}
+
+// ------------------------------------------------------------- walk
+__device__ inline uint32_t uni(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
+__device__ inline double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(v, o, 64);
+        v = (t < v) ? t : v;
+    }
+    return v;
+}
+__device__ inline double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double t = __shfl_xor(v, o, 64);
+        v = (v < t) ? t : v;
+    }
+    return v;
+}
+
+// number of set lanes of mask below this lane
+__device__ inline uint32_t rank_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), o, 64));
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+struct Acc3 {
+    float sx = 0.f, sy = 0.f, sz = 0.f;   // Kahan sums
+    float cx = 0.f, cy = 0.f, cz = 0.f;   // compensations
+    __device__ static void kadd(float& s, float& c, float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ void add(float bx, float by, float bz)
+    {
+        kadd(sx, cx, bx);
+        kadd(sy, cy, by);
+        kadd(sz, cz, bz);
+    }
+};
+
+__device__ inline void interact(const float4 q, float ox, float oy, float oz, float soft2, float& bx, float& by,
+                                float& bz)
+{
+    const float dx = q.x - ox;
+    const float dy = q.y - oy;
+    const float dz = q.z - oz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+    const float inv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+    const float s = q.w * (inv * inv * inv);
+    bx = fmaf(s, dx, bx);
+    by = fmaf(s, dy, by);
+    bz = fmaf(s, dz, bz);
+}
+
+// Evaluates list[0..ln) for this lane's body (offsets ox, oy, oz): FP32 FMA
+// inside blocks of 32 entries, Kahan-compensated across blocks.
+__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, float ox, float oy, float oz,
+                                  float soft2, Acc3& a)
+{
+    ln = uni(ln);
+    uint32_t e0 = 0;
+    for (; e0 + 32u <= ln; e0 += 32u) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+        for (int u = 0; u < 32; u += 8) {
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = list[e0 + u + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) interact(q[k], ox, oy, oz, soft2, bx, by, bz);
+        }
+        a.add(bx, by, bz);
+    }
+    if (e0 < ln) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        for (uint32_t e = e0; e < ln; ++e) interact(list[e], ox, oy, oz, soft2, bx, by, bz);
+        a.add(bx, by, bz);
+    }
+}
+
+constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
+
+__global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
+                                                    MixedParams p)
+{
+    __shared__ float4 s_list[kWaves][kListCap];
+    __shared__ uint32_t s_stack[kWaves][kStackCap];
+    __shared__ uint32_t s_nodes[kWaves][kNodeBuf];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float4* list = s_list[w];
+    uint32_t* stk = s_stack[w];
+    uint32_t* nbuf = s_nodes[w];
+    const uint32_t G = tb.sc->n_groups;
+    const uint32_t worker = blockIdx.x * kWaves + w;
+    const float soft2f = static_cast<float>(p.soft2);
+    const double theta2 = p.theta2;
+    uint32_t pos = 0, end = 0;
+    if (p.sched == 2) {
+        pos = worker < p.n_workers ? mb.chunk[worker] : 0;
+        end = worker < p.n_workers ? mb.chunk[worker + 1] : 0;
+    }
+    for (;;) {
+        uint32_t g;
+        if (p.sched == 2) {
+            if (pos >= end) break;
+            g = mb.order[pos++];
+        } else {
+            uint32_t q = 0;
+            if (lane == 0) q = atomicAdd(&tb.sc->group_counter, 1u);
+            q = uni(q);
+            if (q >= G) break;
+            g = (p.sched == 1) ? mb.order[q] : q;
+        }
+        g = uni(g);
+        if (g >= G) continue;   // a schedule entry outside the groups of this step
+        const uint32_t b0 = mb.gfirst[g];
+        const uint32_t cnt = mb.gfirst[g + 1] - b0;
+        const bool valid = static_cast<uint32_t>(lane) < cnt;
+        const uint32_t i = b0 + (valid ? static_cast<uint32_t>(lane) : 0u);
+        const double x = b.px[i], y = b.py[i], z = b.pz[i];
+        const double lox = wave_min(x), loy = wave_min(y), loz = wave_min(z);
+        const double hix = wave_max(x), hiy = wave_max(y), hiz = wave_max(z);
+        const double cx = 0.5 * (lox + hix), cy = 0.5 * (loy + hiy), cz = 0.5 * (loz + hiz);
+        const double hx = 0.5 * (hix - lox), hy = 0.5 * (hiy - loy), hz = 0.5 * (hiz - loz);
+        const float ox = static_cast<float>(x - cx);
+        const float oy = static_cast<float>(y - cy);
+        const float oz = static_cast<float>(z - cz);
+        Acc3 acc;
+        uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0;
+        bool overflow = false;
+        if (lane == 0) stk[0] = enc(0, 1, false);
+        __builtin_amdgcn_wave_barrier();
+        while (sp > 0 && !overflow) {
+            // pop up to 64 ranges, one per lane, while the stack cannot overflow
+            const uint32_t P = (sp <= static_cast<uint32_t>(kBigPopLimit)) ? min(sp, 64u) : 1u;
+            const bool has = static_cast<uint32_t>(lane) < P;
+            const uint32_t e = has ? stk[sp - 1 - lane] : 0u;
+            sp -= P;
+            const uint32_t ecount = has ? ((e >> 27) & 7u) + 1u : 0u;
+            const uint32_t incl = wave_incl_scan(ecount);
+            const uint32_t total = uni(static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64)));
+            __builtin_amdgcn_wave_barrier();
+            {
+                const uint32_t kind = e & 0x80000000u;
+                const uint32_t first = e & 0x07ffffffu;
+                for (uint32_t k = 0, o = incl - ecount; k < ecount; ++k) nbuf[o + k] = (first + k) | kind;
+            }
+            __builtin_amdgcn_wave_barrier();
+            n_visit += total;
+            for (uint32_t base = 0; base < total; base += 64u) {
+                const uint32_t s = base + static_cast<uint32_t>(lane);
+                const bool act = s < total;
+                const uint32_t ent = act ? nbuf[s] : 0u;
+                bool app = false;
+                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint32_t npush = 0, pfirst = 0, pcount = 0;
+                bool pbody = false;
+                if (act) {
+                    const uint32_t id = ent & 0x7fffffffu;
+                    if (ent & 0x80000000u) {
+                        const double mb_ = b.m[id];
+                        if (b.hrank[id] < 0 && mb_ != 0.0) {
+                            app = true;
+                            q = make_float4(static_cast<float>(b.px[id] - cx), static_cast<float>(b.py[id] - cy),
+                                            static_cast<float>(b.pz[id] - cz), static_cast<float>(p.G * mb_));
+                        }
+                    } else {
+                        const MNode nd = mb.node[id];
+                        if (nd.gm != 0.f) {
+                            const uint32_t nc = nd.info & ~kMLeaf;
+                            if (nd.info & kMLeaf) {
+                                if (nc == 1) {
+                                    const uint32_t bi = nd.first;
+                                    app = true;
+                                    q = make_float4(static_cast<float>(b.px[bi] - cx), static_cast<float>(b.py[bi] - cy),
+                                                    static_cast<float>(b.pz[bi] - cz),
+                                                    static_cast<float>(p.G * b.m[bi]));
+                                } else {
+                                    npush = (nc + 7u) >> 3;
+                                    pfirst = nd.first;
+                                    pcount = nc;
+                                    pbody = true;
+                                }
+                            } else {
+                                const double ex = fmax(fabs(nd.x - cx) - hx, 0.0);
+                                const double ey = fmax(fabs(nd.y - cy) - hy, 0.0);
+                                const double ez = fmax(fabs(nd.z - cz) - hz, 0.0);
+                                const double d2 = ex * ex + ey * ey + ez * ez;
+                                if (nd.side2 < theta2 * d2) {
+                                    app = true;
+                                    q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
+                                                    static_cast<float>(nd.z - cz), nd.gm);
+                                } else {
+                                    npush = 1;
+                                    pfirst = nd.first;
+                                    pcount = nc;
+                                }
+                            }
+                        }
+                    }
+                }
+                // interaction list appends (one per lane at most)
+                const uint64_t am = __ballot(app);
+                const uint32_t na = static_cast<uint32_t>(__popcll(am));
+                if (ln + na > static_cast<uint32_t>(kListCap)) {
+                    __builtin_amdgcn_wave_barrier();
+                    flush_list(list, ln, ox, oy, oz, soft2f, acc);
+                    __builtin_amdgcn_wave_barrier();
+                    ln = 0;
+                }
+                if (app) list[ln + rank_below(am)] = q;
+                ln += na;
+                n_inter += na;
+                // stack pushes: one range per opened node; a leaf of more than
+                // 8 co-keyed bodies pushes one body range per 8 bodies
+                const uint64_t pm = __ballot(npush == 1u);
+                uint64_t big = __ballot(npush > 1u);
+                uint32_t need = static_cast<uint32_t>(__popcll(pm));
+                for (uint64_t bm = big; bm; bm &= bm - 1) {
+                    const int l = __builtin_ctzll(bm);
+                    need += uni(static_cast<uint32_t>(__shfl(static_cast<int>(npush), l, 64)));
+                }
+                if (sp + need > static_cast<uint32_t>(kStackCap)) {
+                    if (lane == 0) atomicOr(&tb.sc->error, kErrStack);
+                    overflow = true;
+                    break;
+                }
+                if (npush == 1u) stk[sp + rank_below(pm)] = enc(pfirst, pcount, pbody);
+                sp += static_cast<uint32_t>(__popcll(pm));
+                for (; big; big &= big - 1) {
+                    const int l = __builtin_ctzll(big);
+                    const uint32_t f = uni(static_cast<uint32_t>(__shfl(static_cast<int>(pfirst), l, 64)));
+                    const uint32_t c = uni(static_cast<uint32_t>(__shfl(static_cast<int>(pcount), l, 64)));
+                    const uint32_t ne = (c + 7u) >> 3;
+                    for (uint32_t k = static_cast<uint32_t>(lane); k < ne; k += 64u)
+                        stk[sp + k] = enc(f + 8u * k, min(c - 8u * k, 8u), true);
+                    sp += ne;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        flush_list(list, ln, ox, oy, oz, soft2f, acc);
+        __builtin_amdgcn_wave_barrier();
+        // heavy bodies: FP64 direct sum in rank order
+        double hax = 0.0, hay = 0.0, haz = 0.0;
+        for (int k = 0; k < p.n_heavy; ++k) {
+            const uint32_t j = tb.heavy_slot[k];
+            if (j == i) continue;
+            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
+            if (!(r2 > 0.0)) continue;
+            const double invR = 1.0 / sqrt(r2);
+            const double s = p.G * b.m[j] * (invR * invR * invR);
+            hax += s * dx;
+            hay += s * dy;
+            haz += s * dz;
+        }
+        if (valid) {
+            const double ax = hax + (static_cast<double>(acc.sx) + static_cast<double>(acc.cx));
+            const double ay = hay + (static_cast<double>(acc.sy) + static_cast<double>(acc.cy));
+            const double az = haz + (static_cast<double>(acc.sz) + static_cast<double>(acc.cz));
+            b.ax[i] = ax;
+            b.ay[i] = ay;
+            b.az[i] = az;
+            if (p.kick) {
+                b.vx[i] = b.vx[i] + p.half_dt * ax;
+                b.vy[i] = b.vy[i] + p.half_dt * ay;
+                b.vz[i] = b.vz[i] + p.half_dt * az;
+            }
+            b.pcost[i] = n_inter;
+        }
+        if (lane == 0) {
+            mb.g_inter[g] = n_inter;
+            mb.g_visit[g] = n_visit;
+        }
+    }
+}
+
+// ------------------------------------------------------------ schedule
+// Longest-processing-time order of the groups for the dynamic queue: groups
+// go into 64 buckets of descending estimated cost (two buckets per octave)
+// and the queue serves the costliest bucket first.  The order inside a bucket
+// is whatever the atomics give: it changes timing only, never results.
+//
+// Cost estimate, source 0: the previous step, i.e. the largest list length
+// any of the group's bodies saw (groups mostly persist from step to step).
+// Source 1: mb.gcost, written by the cost model.
+__device__ inline uint32_t cost_bucket(float c)
+{
+    const float l = 2.f * __log2f(c + 1.f);
+    const int k = static_cast<int>(l);
+    return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
+}
+
+__global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, int source)
+{
+    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t G = tb.sc->n_groups;
+    if (g >= G) return;
+    float c;
+    if (source == 0) {
+        uint32_t m = 0;
+        for (uint32_t i = mb.gfirst[g]; i < mb.gfirst[g + 1]; ++i) m = max(m, b.pcost[i]);
+        c = static_cast<float>(m);
+        mb.gcost[g] = c;
+    } else {
+        c = mb.gcost[g];
+    }
+    atomicAdd(&mb.sched_hist[cost_bucket(c)], 1u);
+}
+
+__global__ __launch_bounds__(64) void k_sched_scan(MixedBuffers mb)
+{
+    // exclusive scan of the 64 bucket counts; the counts are left at zero
+    const int k = threadIdx.x;
+    const uint32_t v = mb.sched_hist[k];
+    const uint32_t inc = wave_incl_scan(v);
+    mb.sched_cursor[k] = inc - v;
+    mb.sched_hist[k] = 0;
+}
+
+__global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb)
+{
+    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (g >= tb.sc->n_groups) return;
+    const uint32_t pos = atomicAdd(&mb.sched_cursor[cost_bucket(mb.gcost[g])], 1u);
+    mb.order[pos] = static_cast<uint32_t>(g);
+}
+
+inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kNT > 0 ? (n + kNT - 1) / kNT : 1); }
+
+}  // namespace
+
+void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s)
+{
+    k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb);
+    scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
+    k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, mb.order);
+    scan_exclusive_u32(mb.order, mb.base, nullptr, n, tb.scan_tmp, s);
+    k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+    k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, G);
+}
+
+unsigned mixed_walk_blocks(int device)
+{
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_walk_mixed), kNT, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return static_cast<unsigned>(cus * per_cu);
+}
+
+void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
+                     hipStream_t s)
+{
+    k_sched_cost<<<nblk(n), kNT, 0, s>>>(n, b, tb, mb, source);
+    k_sched_scan<<<1, 64, 0, s>>>(mb);
+    k_sched_scatter<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+}
+
+void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
+                       const MixedParams& p, unsigned blocks, hipStream_t s)
+{
+    if (n == 0 || blocks == 0) return;
+    k_walk_mixed<<<blocks, kNT, 0, s>>>(b, n, tb, mb, p);
+}
+
+}  // namespace bh

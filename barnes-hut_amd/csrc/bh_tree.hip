@@ -142,13 +142,39 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int 
     __syncthreads();
     if (!last) return;
     __threadfence();
+    // the last block reduces every block's partial box in parallel
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += kNT) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = dmin(lo[a], __hip_atomic_load(&partials[6 * k + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            hi[a] = dmax(hi[a], __hip_atomic_load(&partials[6 * k + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
+            hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
+        }
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            red[a][w] = lo[a];
+            red[3 + a][w] = hi[a];
+        }
+    __syncthreads();
     if (threadIdx.x == 0) {
         double bb[6];
         for (int a = 0; a < 3; ++a) {
-            double l = INFINITY, h = -INFINITY;
-            for (unsigned k = 0; k < gridDim.x; ++k) {
-                l = dmin(l, __hip_atomic_load(&partials[6 * k + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                h = dmax(h, __hip_atomic_load(&partials[6 * k + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            double l = red[a][0], h = red[3 + a][0];
+            for (int k = 1; k < kNT / 64; ++k) {
+                l = dmin(l, red[a][k]);
+                h = dmax(h, red[3 + a][k]);
             }
             bb[a] = l;
             bb[3 + a] = h;
@@ -166,6 +192,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int 
             sc->side2[d] = side * side;
         }
         sc->error = 0;
+        sc->group_counter = 0;
         sc->block_counter = 0;
     }
 }
@@ -206,11 +233,15 @@ __global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, const StepScal
 // ---------------------------------------------------------- permute
 __global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, const uint32_t* __restrict__ perm,
                                                  const uint64_t* __restrict__ key, uint32_t* __restrict__ flag,
-                                                 size_t n)
+                                                 uint32_t* __restrict__ heavy_slot, size_t n)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = perm[i];
+    const int8_t hr = src.hrank[j];
+    dst.hrank[i] = hr;
+    dst.pcost[i] = src.pcost[j];
+    if (hr >= 0) heavy_slot[hr] = static_cast<uint32_t>(i);
     dst.px[i] = src.px[j];
     dst.py[i] = src.py[j];
     dst.pz[i] = src.pz[j];
@@ -232,6 +263,7 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
     const uint32_t f = flag[i];
     const uint32_t j = excl[i] + f - 1;
     tb.leaf_of[i] = j;
+    if (tb.gflag) tb.gflag[i] = 0;
     if (f) {
         tb.ukey[j] = key[i];
         tb.leaf_first[j] = static_cast<uint32_t>(i);
@@ -262,6 +294,23 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
         r[1] = sx;
         r[2] = sy;
         r[3] = sz;
+        if (tb.lraw) {
+            // moments without the heavy bodies, which the mixed walk sums directly
+            double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+            for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
+                if (b.hrank[i] >= 0) continue;
+                const double mi = b.m[i];
+                lm += mi;
+                lx += b.px[i] * mi;
+                ly += b.py[i] * mi;
+                lz += b.pz[i] * mi;
+            }
+            double* q = tb.lraw + 4 * j;
+            q[0] = lm;
+            q[1] = lx;
+            q[2] = ly;
+            q[3] = lz;
+        }
     }
     uint32_t isrep = 0;
     if (M > 1 && j < M - 1) {
@@ -351,6 +400,24 @@ __global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb,
     r[1] = sx;
     r[2] = sy;
     r[3] = sz;
+    if (tb.lraw) {
+        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int32_t ck = c[k];
+            if (ck < 0) continue;
+            const double* q = tb.lraw + 4 * static_cast<size_t>(ck);
+            lm += q[0];
+            lx += q[1];
+            ly += q[2];
+            lz += q[3];
+        }
+        double* q = tb.lraw + 4 * (static_cast<size_t>(M) + bi);
+        q[0] = lm;
+        q[1] = lx;
+        q[2] = ly;
+        q[3] = lz;
+    }
 }
 
 // reversed per-leaf node counts for the suffix scan
@@ -441,7 +508,7 @@ void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt,
                            StepScalars* sc, double* partials, hipStream_t s)
 {
     unsigned g = nblk(n);
-    if (g > 1024) g = 1024;
+    if (g > 512) g = 512;   // partials hold 1024 blocks
     k_kick_drift_box<<<g, kNT, 0, s>>>(b, n, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, sc, partials);
 }
 
@@ -452,9 +519,9 @@ void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key
 }
 
 void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
-                    uint32_t* flag, size_t n, hipStream_t s)
+                    uint32_t* flag, uint32_t* heavy_slot, size_t n, hipStream_t s)
 {
-    k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, perm, key, flag, n);
+    k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, perm, key, flag, heavy_slot, n);
 }
 
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s)
@@ -467,6 +534,10 @@ void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s
     k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
     k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
     for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(n, tb, d);
+}
+
+void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s)
+{
     k_suffix_input<<<nblk(n), kNT, 0, s>>>(n, tb, tb.is_rep);
     scan_exclusive_u32(tb.is_rep, tb.suffix, &tb.sc->n_nodes, n, tb.scan_tmp, s);
     k_walk_records<<<nblk(2 * n), kNT, 0, s>>>(n, tb);

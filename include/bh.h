@@ -142,6 +142,25 @@ int bh_get_phase_ms(bh_ctx *ctx, double *ms5);
  * bh_step call (at most the last 1024 steps); *nsteps gets the count. */
 int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
 
+/* Order in which the persistent mixed walk serves its groups (results never
+ * depend on it):
+ *   KEY_ORDER  dynamic queue in Morton order (default of MIXED)
+ *   PREV_COST  heaviest first by the list lengths measured in the last step
+ *   MODEL      heaviest first by the learned cost model (LEARNED; falls back
+ *              to PREV_COST while no model is trained) */
+#define BH_SCHED_KEY_ORDER 0
+#define BH_SCHED_PREV_COST 1
+#define BH_SCHED_MODEL 2
+int bh_set_schedule(bh_ctx *ctx, int mode);
+
+/* Walk groups of the last MIXED / LEARNED evaluation: up to 64
+ * Morton-consecutive bodies walked together.  *n_groups gets the count; when
+ * capacity >= count, first_body[g] (count + 1 entries, last = n),
+ * interactions[g] (list entries evaluated) and visited[g] (nodes tested)
+ * are filled.  Any array may be NULL. */
+int bh_get_groups(bh_ctx *ctx, int64_t *n_groups, int64_t capacity, uint32_t *first_body,
+                  uint32_t *interactions, uint32_t *visited);
+
 /* FP64 direct sum on the device for m sample bodies (indices into the
  * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 with
  * Neumaier-compensated sums, r^2 = |d|^2 + soft2.  The reference for force

@@ -36,7 +36,9 @@ def gpu_force(engine, pos, vel, mass, theta, tree=False, counts=False):
     engine.set_counting(counts)
     acc = engine.compute_accelerations(theta=theta)
     st = engine.get_bodies()
-    out = dict(acc=acc, keys=engine.keys(), bb=engine.bbox(), **st)
+    assert np.array_equal(bits(st["acc"]), bits(acc))
+    out = dict(st)
+    out.update(keys=engine.keys(), bb=engine.bbox())
     if tree:
         out["tree"] = engine.topology()
     if counts:
