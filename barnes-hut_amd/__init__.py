@@ -77,6 +77,15 @@ SIGNATURES = [
     ("bh_direct_sum", C.c_int, [_P, _I64, _P, _D, _D, _P]),
     ("bh_get_groups", C.c_int, [_P, C.POINTER(_I64), _I64, _P, _P, _P]),
     ("bh_set_schedule", C.c_int, [_P, C.c_int]),
+    ("bh_model_configure", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_float]),
+    ("bh_model_reset", C.c_int, [_P, C.c_uint64]),
+    ("bh_model_get_params", C.c_int, [_P, _P]),
+    ("bh_model_set_params", C.c_int, [_P, _P]),
+    ("bh_model_forward", C.c_int, [_P, _I64, _P, _P]),
+    ("bh_model_gradients", C.c_int, [_P, _I64, _P, _P, _P, C.POINTER(C.c_float)]),
+    ("bh_model_train", C.c_int, [_P, _I64, _P, _P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    ("bh_model_status", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_float)]),
+    ("bh_get_group_features", C.c_int, [_P, _I64, _P]),
     ("bh_generate", C.c_int, [C.c_int, C.c_uint64, _I64, _P, _P, _P]),
     ("bh_generate_ensemble_sizes", C.c_int, [C.c_uint64, _I64, _I64, _I64, _P]),
     ("bh_write_csv", C.c_int, [C.c_char_p, _I64, _P, _P, _P]),
@@ -285,6 +294,57 @@ class Engine:
 
     def set_schedule(self, mode: int):
         self._ck(self.lib.bh_set_schedule(self.h, mode))
+
+    # ------------------------------------------------ learned cost model
+    MODEL_PARAMS = 1377
+
+    def model_configure(self, train_steps=50, iters_per_step=8, batch=2048, lr=3e-3):
+        self._ck(self.lib.bh_model_configure(self.h, train_steps, iters_per_step, batch, lr))
+
+    def model_reset(self, seed: int):
+        self._ck(self.lib.bh_model_reset(self.h, seed))
+
+    def model_get_params(self) -> np.ndarray:
+        out = np.zeros(self.MODEL_PARAMS, np.float32)
+        self._ck(self.lib.bh_model_get_params(self.h, _ptr(out)))
+        return out
+
+    def model_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._ck(self.lib.bh_model_set_params(self.h, _ptr(p)))
+
+    def model_forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.zeros(len(x), np.float32)
+        self._ck(self.lib.bh_model_forward(self.h, len(x), _ptr(x), _ptr(out)))
+        return out
+
+    def model_gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.MODEL_PARAMS, np.float32)
+        loss = C.c_float(0)
+        self._ck(self.lib.bh_model_gradients(self.h, len(x), _ptr(x), _ptr(t), _ptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def model_train(self, x, t, iters=100, batch=1024, lr=3e-3) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_float(0)
+        self._ck(self.lib.bh_model_train(self.h, len(x), _ptr(x), _ptr(t), iters, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def model_status(self):
+        r, s, u, l = C.c_int(0), _I64(0), _I64(0), C.c_float(0)
+        self._ck(self.lib.bh_model_status(self.h, C.byref(r), C.byref(s), C.byref(u), C.byref(l)))
+        return bool(r.value), s.value, u.value, l.value
+
+    def group_features(self) -> np.ndarray:
+        k = _I64(0)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
+        out = np.zeros((k.value, 8), np.float32)
+        self._ck(self.lib.bh_get_group_features(self.h, k.value, _ptr(out)))
+        return out
 
     def groups(self):
         """Walk groups of the last mixed evaluation: first_body (G+1),

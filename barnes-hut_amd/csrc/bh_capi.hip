@@ -22,6 +22,7 @@
 
 using bh::Bodies;
 using bh::MixedBuffers;
+using bh::ModelBuffers;
 using bh::StepScalars;
 using bh::TreeBuffers;
 using bh::WalkNode;
@@ -39,6 +40,14 @@ struct bh_ctx {
     unsigned mixed_blocks = 0;
     int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
     bool model_ready = false;         // the LEARNED cost model is trained
+    // cost model (allocated once per context, independent of N)
+    ModelBuffers md{};
+    std::vector<void*> model_allocs;
+    int train_steps = 50, iters_per_step = 16, batch = 2048;
+    float lr = 3e-3f;
+    int64_t trained_steps = 0;
+    int64_t updates = 0;
+    bool have_features = false;
     double* partials = nullptr;
     int64_t* cnt_a = nullptr;
     int64_t* cnt_o = nullptr;
@@ -85,7 +94,9 @@ void free_buffers(bh_ctx* c)
     c->cur = Bodies{};
     c->alt = Bodies{};
     c->tb = TreeBuffers{};
-    c->mb = MixedBuffers{};
+    MixedBuffers keep{};
+    c->mb = keep;
+    c->md.feat = nullptr;
     c->partials = nullptr;
     c->cnt_a = c->cnt_o = c->cnt_t = nullptr;
     c->cap = 0;
@@ -126,12 +137,13 @@ int reserve(bh_ctx* c, size_t n)
     A(t.ukey, N); A(t.leaf_first, N + 1); A(t.delta, N); A(t.rep, N); A(t.is_rep, N);
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
-    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.walk, 2 * N);
+    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.walk, 2 * N);
     A(t.heavy_slot, 64); A(t.gflag, N);
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
     A(m.g_inter, N); A(m.g_visit, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
-    A(m.sched_hist, 64); A(m.sched_cursor, 64);
+    A(m.sched_hist, 64); A(m.sched_cursor, 64); A(m.hpart, 3 * bh::kMaxHeavy * bh::kHeavyChunks);
+    A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
     HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
     A(t.sc, 1);
@@ -144,7 +156,7 @@ int reserve(bh_ctx* c, size_t n)
 
 // One force evaluation with optional leading kick + drift and trailing kick.
 constexpr int kRing = 512;
-constexpr int kHeavy = 16;
+constexpr int kHeavy = bh::kMaxHeavy;
 
 int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool kick_before, bool drift,
               double dt, bool kick_after, const hipEvent_t* ev)
@@ -174,13 +186,28 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     if (mixed) {
         bh::launch_mixed_prepare(c->n, tb, c->mb, G, s);
         int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;
+        const bool learned = form == BH_FORM_LEARNED;
+        if (learned || sched == BH_SCHED_MODEL) {
+            bh::launch_group_features(c->n, c->cur, tb, c->mb, c->md.feat, s);
+            c->have_features = true;
+        }
         if (sched == BH_SCHED_MODEL && !c->model_ready) sched = BH_SCHED_PREV_COST;
+        if (sched == BH_SCHED_MODEL) bh::launch_model_cost(c->n, c->md.feat, c->md, tb, c->mb, s);
         if (sched != BH_SCHED_KEY_ORDER)
             bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, s);
         HIP_TRY(c, hipEventRecord(ev[3], s));
         bh::MixedParams mp{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0, c->n_heavy,
                            sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4};
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
+        if (learned && c->trained_steps < c->train_steps) {
+            // training phase: the walk's list lengths become samples, then
+            // iters_per_step Adam updates on mini-batches of all samples so far
+            bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
+            for (int k = 0; k < c->iters_per_step; ++k)
+                bh::launch_model_train(c->md, nullptr, nullptr, 0, c->batch, 1, static_cast<uint32_t>(c->updates++),
+                                       c->lr, true, nullptr, nullptr, s);
+            if (++c->trained_steps >= c->train_steps) c->model_ready = true;
+        }
         HIP_TRY(c, hipEventRecord(ev[4], s));
         HIP_TRY(c, hipGetLastError());
         c->last_ev = ev;
@@ -210,6 +237,37 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
 }
 
 int check_ctx(const bh_ctx* c) { return c ? BH_OK : BH_ERR_ARG; }
+
+constexpr uint64_t kSampleCapacity = 1u << 20;
+constexpr int64_t kModelIoRows = 1 << 16;   // rows per host model call
+
+template <class T>
+int malloc_model(bh_ctx* c, T** p, size_t count)
+{
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, BH_ERR_NOMEM, "hipMalloc(model): %s", hipGetErrorString(e));
+    c->model_allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return BH_OK;
+}
+
+int model_alloc(bh_ctx* c)
+{
+    ModelBuffers& d = c->md;
+    int rc;
+    if ((rc = malloc_model(c, &d.wf32, bh::kModelParams))) return rc;
+    if ((rc = malloc_model(c, &d.wbf, bh::kModelParams))) return rc;
+    if ((rc = malloc_model(c, &d.adam_m, bh::kModelParams))) return rc;
+    if ((rc = malloc_model(c, &d.adam_v, bh::kModelParams))) return rc;
+    if ((rc = malloc_model(c, &d.sfeat, bh::kModelFeatures * kSampleCapacity))) return rc;
+    if ((rc = malloc_model(c, &d.starget, kSampleCapacity))) return rc;
+    if ((rc = malloc_model(c, &d.n_seen, 1))) return rc;
+    if ((rc = malloc_model(c, &d.last_loss, 1))) return rc;
+    if ((rc = malloc_model(c, &d.io, (bh::kModelFeatures + 1) * kModelIoRows + bh::kModelParams + 1))) return rc;
+    d.capacity = kSampleCapacity;
+    return BH_OK;
+}
 
 // Error bits raised by device-side guards during the last evaluation.
 int check_device_errors(bh_ctx* c)
@@ -287,6 +345,13 @@ int bh_create(bh_ctx** out, const bh_config* cfg)
             return fail(nullptr, BH_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
         }
     }
+    int rc = model_alloc(c);
+    if (rc == BH_OK) rc = bh_model_reset(c, 0x5eed);
+    if (rc != BH_OK) {
+        g_create_error = c->err;
+        bh_destroy(c);
+        return rc;
+    }
     *out = c;
     return BH_OK;
 }
@@ -297,6 +362,7 @@ void bh_destroy(bh_ctx* c)
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_buffers(c);
+    for (void* p : c->model_allocs) (void)hipFree(p);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : c->ring)
@@ -595,7 +661,7 @@ int phase_ms(bh_ctx* c, const hipEvent_t* ev, double* ms5)
 int bh_get_phase_ms(bh_ctx* c, double* ms5)
 {
     if (check_ctx(c) || !ms5) return BH_ERR_ARG;
-    if (!c->have_tree || !c->last_ev) return fail(c, BH_ERR_STATE, "no force evaluation yet");
+    if (!c->last_ev) return fail(c, BH_ERR_STATE, "no force evaluation yet");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return phase_ms(c, c->last_ev, ms5);
 }
@@ -673,6 +739,157 @@ int bh_direct_sum(bh_ctx* c, int64_t m, const int64_t* idx, double G, double sof
     (void)hipFree(d_idx);
     (void)hipFree(d_acc);
     return rc;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------- cost model
+extern "C" {
+
+int bh_model_configure(bh_ctx* c, int train_steps, int iters_per_step, int batch, float lr)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (train_steps < 0 || iters_per_step < 0 || batch <= 0 || !(lr > 0.f))
+        return fail(c, BH_ERR_ARG, "bad model configuration");
+    c->train_steps = train_steps;
+    c->iters_per_step = iters_per_step;
+    c->batch = batch;
+    c->lr = lr;
+    return BH_OK;
+}
+
+int bh_model_set_params(bh_ctx* c, const float* params)
+{
+    if (check_ctx(c) || !params) return BH_ERR_ARG;
+    HIP_TRY(c, hipMemcpyAsync(c->md.wf32, params, bh::kModelParams * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->md.adam_m, 0, bh::kModelParams * 4, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->md.adam_v, 0, bh::kModelParams * 4, c->stream));
+    bh::launch_refresh_bf16(c->md, c->stream);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->updates = 0;
+    return BH_OK;
+}
+
+int bh_model_reset(bh_ctx* c, uint64_t seed)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    // He-uniform weights for the ReLU layers, a small output layer, zero biases
+    std::vector<float> w(bh::kModelParams, 0.f);
+    uint64_t z = seed;
+    auto uni = [&]() {
+        z += 0x9e3779b97f4a7c15ULL;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        x ^= x >> 31;
+        return static_cast<double>(x >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+    };
+    const int F = bh::kModelFeatures, H = 32;
+    for (int k = 0; k < H * F; ++k) w[k] = static_cast<float>(uni() * std::sqrt(6.0 / F));
+    for (int k = 0; k < H * H; ++k) w[H * F + H + k] = static_cast<float>(uni() * std::sqrt(6.0 / H));
+    for (int k = 0; k < H; ++k) w[H * F + H + H * H + H + k] = static_cast<float>(uni() * std::sqrt(1.0 / H));
+    int rc = bh_model_set_params(c, w.data());
+    if (rc) return rc;
+    HIP_TRY(c, hipMemsetAsync(c->md.n_seen, 0, 8, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->md.last_loss, 0, 4, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->model_ready = false;
+    c->trained_steps = 0;
+    return BH_OK;
+}
+
+int bh_model_get_params(bh_ctx* c, float* params)
+{
+    if (check_ctx(c) || !params) return BH_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(params, c->md.wf32, bh::kModelParams * 4, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+int bh_model_forward(bh_ctx* c, int64_t m, const float* features, float* out)
+{
+    if (check_ctx(c) || m < 0 || (m > 0 && (!features || !out))) return BH_ERR_ARG;
+    float* dx = c->md.io;
+    float* dy = dx + bh::kModelFeatures * kModelIoRows;
+    for (int64_t r0 = 0; r0 < m; r0 += kModelIoRows) {
+        const int64_t r = std::min<int64_t>(kModelIoRows, m - r0);
+        HIP_TRY(c, hipMemcpyAsync(dx, features + bh::kModelFeatures * r0, r * bh::kModelFeatures * 4,
+                                  hipMemcpyHostToDevice, c->stream));
+        bh::launch_model_forward(dx, static_cast<size_t>(r), c->md, dy, c->stream);
+        HIP_TRY(c, hipMemcpyAsync(out + r0, dy, r * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return BH_OK;
+}
+
+int bh_model_gradients(bh_ctx* c, int64_t m, const float* features, const float* targets, float* grad, float* loss)
+{
+    if (check_ctx(c) || m <= 0 || m > kModelIoRows || !features || !targets || !grad)
+        return fail(c, BH_ERR_ARG, "bh_model_gradients takes 1..%lld rows", (long long)kModelIoRows);
+    float* dx = c->md.io;
+    float* dt = dx + bh::kModelFeatures * kModelIoRows;
+    float* dg = dt + kModelIoRows;
+    float* dl = dg + bh::kModelParams;
+    HIP_TRY(c, hipMemcpyAsync(dx, features, m * bh::kModelFeatures * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dt, targets, m * 4, hipMemcpyHostToDevice, c->stream));
+    bh::launch_model_train(c->md, dx, dt, static_cast<uint64_t>(m), static_cast<int>(m), 0, 0, 0.f, false, dg, dl,
+                           c->stream);
+    HIP_TRY(c, hipMemcpyAsync(grad, dg, bh::kModelParams * 4, hipMemcpyDeviceToHost, c->stream));
+    float l = 0.f;
+    HIP_TRY(c, hipMemcpyAsync(&l, dl, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return BH_OK;
+}
+
+int bh_model_train(bh_ctx* c, int64_t m, const float* features, const float* targets, int iters, int batch,
+                   float lr, float* final_loss)
+{
+    if (check_ctx(c) || m <= 0 || m > kModelIoRows || !features || !targets || iters < 0 || batch <= 0 || !(lr > 0))
+        return fail(c, BH_ERR_ARG, "bh_model_train takes 1..%lld rows", (long long)kModelIoRows);
+    float* dx = c->md.io;
+    float* dt = dx + bh::kModelFeatures * kModelIoRows;
+    float* dl = dt + kModelIoRows + bh::kModelParams;
+    HIP_TRY(c, hipMemcpyAsync(dx, features, m * bh::kModelFeatures * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(dt, targets, m * 4, hipMemcpyHostToDevice, c->stream));
+    for (int k = 0; k < iters; ++k)
+        bh::launch_model_train(c->md, dx, dt, static_cast<uint64_t>(m), batch, 1,
+                               static_cast<uint32_t>(c->updates++), lr, true, nullptr, dl, c->stream);
+    HIP_TRY(c, hipGetLastError());
+    float l = 0.f;
+    HIP_TRY(c, hipMemcpyAsync(&l, dl, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (final_loss) *final_loss = l;
+    c->model_ready = true;
+    return BH_OK;
+}
+
+int bh_model_status(bh_ctx* c, int* ready, int64_t* samples, int64_t* updates, float* last_loss)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    uint64_t seen = 0;
+    float l = 0.f;
+    HIP_TRY(c, hipMemcpy(&seen, c->md.n_seen, 8, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&l, c->md.last_loss, 4, hipMemcpyDeviceToHost));
+    if (ready) *ready = c->model_ready ? 1 : 0;
+    if (samples) *samples = static_cast<int64_t>(seen);
+    if (updates) *updates = c->updates;
+    if (last_loss) *last_loss = l;
+    return BH_OK;
+}
+
+int bh_get_group_features(bh_ctx* c, int64_t capacity, float* features)
+{
+    if (check_ctx(c) || !features) return BH_ERR_ARG;
+    if (!c->have_features) return fail(c, BH_ERR_STATE, "no LEARNED evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    if (capacity < static_cast<int64_t>(sc.n_groups)) return fail(c, BH_ERR_OVERFLOW, "%u groups", sc.n_groups);
+    HIP_TRY(c, hipMemcpy(features, c->md.feat, static_cast<size_t>(sc.n_groups) * bh::kModelFeatures * 4,
+                         hipMemcpyDeviceToHost));
+    return BH_OK;
 }
 
 }  // extern "C"

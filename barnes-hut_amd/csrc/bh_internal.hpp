@@ -112,6 +112,7 @@ struct TreeBuffers {
     uint32_t* windex;     // walk index
     double* raw;          // 4 per node: mass, sum m*x, sum m*y, sum m*z
     double* lraw;         // the same over light bodies only (mixed forms), or null
+    double* lbox;         // 6 per node: bounding box of the light bodies (min xyz, max xyz)
     uint32_t* heavy_slot; // sorted position of each heavy body, by rank
     uint32_t* gflag;      // walk-group start flags (mixed forms), cleared per step
     WalkNode* walk;       // capacity 2N
@@ -148,7 +149,8 @@ void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const W
 // those of the light bodies; the heavy bodies are summed directly in FP64.
 struct MNode {
     double x, y, z;   // centre of mass of the light bodies (FP64)
-    double side2;     // (L / 2^d)^2 at the chain bottom d (branching nodes)
+    double size2;     // opening size^2: squared distance from the centre of mass
+                      // to the farthest corner of the light bodies' box
     float gm;         // G * light mass
     uint32_t first;   // first child slot (branching) or first body (leaf)
     uint32_t info;    // child or body count | kMLeaf
@@ -157,6 +159,8 @@ struct MNode {
 static_assert(sizeof(MNode) == 48, "MNode is 48 bytes");
 constexpr uint32_t kMLeaf = 1u << 31;
 constexpr int kGroupMax = 64;   // bodies per walk group (one wavefront)
+constexpr int kMaxHeavy = 16;   // bodies summed directly in FP64 by the mixed forms
+constexpr int kHeavyChunks = 64;
 
 struct MixedBuffers {
     MNode* node;        // 2N
@@ -170,6 +174,7 @@ struct MixedBuffers {
     uint32_t* order;    // N: group order of the scheduled walk
     uint32_t* chunk;    // workers + 1: static chunk starts into order[]
     float* gcost;       // N: estimated cost of each group
+    double* hpart;      // heavy direct sums: kMaxHeavy x kHeavyChunks x 3 partials
     uint32_t* sched_hist;    // 64 cost buckets (left at zero)
     uint32_t* sched_cursor;  // 64
 };
@@ -192,6 +197,36 @@ unsigned mixed_walk_blocks(int device);
 // lengths; 1: mb.gcost from the cost model).
 void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
                      hipStream_t s);
+
+// ------------------------------------------------------- cost model
+constexpr int kModelFeatures = 8;
+constexpr int kModelParams = 32 * 8 + 32 + 32 * 32 + 32 + 32 + 1;   // 1377
+
+struct ModelBuffers {
+    float* wf32;          // FP32 master weights
+    uint16_t* wbf;        // BF16 weights used by every forward pass
+    float* adam_m;
+    float* adam_v;
+    float* sfeat;         // ring buffer of samples: capacity x 8 features
+    float* starget;       // capacity targets log2(list length + 1) / 16
+    uint64_t* n_seen;     // samples appended so far (device counter)
+    uint64_t capacity;
+    float* last_loss;     // loss of the last training pass (device)
+    float* feat;          // features of the groups of the current step (N x 8)
+    float* io;            // staging for host calls: (m x 8) + m + 1377 + 1
+};
+
+void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
+                           hipStream_t s);
+void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
+                       const MixedBuffers& mb, hipStream_t s);
+void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s);
+void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, const MixedBuffers& mb,
+                           const ModelBuffers& md, hipStream_t s);
+void launch_model_train(const ModelBuffers& md, const float* x, const float* t, uint64_t m, int batch,
+                        int sample_mode, uint32_t iter, float lr, bool update, float* grad_out, float* loss_out,
+                        hipStream_t s);
+void launch_refresh_bf16(const ModelBuffers& md, hipStream_t s);
 
 // FP64 direct sum for m sample bodies (Neumaier-compensated).
 void launch_direct_sum(const Bodies& b, size_t n, const int64_t* idx, size_t m, double G, double soft2,

@@ -7,12 +7,16 @@
 // 64; a leaf with more than 64 co-keyed bodies is split.  One wavefront walks
 // the tree once per group, one lane per body.
 //
-// Opening test per group: a node is accepted when side(d)^2 < theta^2 * D^2,
-// D the FP64 distance from the node's centre of mass to the group's bounding
-// box.  Every body of the group is at least D from the centre of mass, so the
-// group test implies the reference's per-body test (src/traversal.cpp:78) for
-// all of them.  The result of a group depends only on the group, never on
-// which worker walks it or when: every schedule gives identical bits.
+// Opening test per group: a node is accepted when b^2 < theta^2 * D^2, with b
+// the distance from the node's centre of mass to the farthest corner of the
+// bounding box of its light bodies (the node's true extent, <= the cell
+// diagonal) and D the FP64 distance from that centre of mass to the group's
+// bounding box.  Every body of the group is at least D away, so each body
+// sees the node inside a cone of half-angle ~theta, the same accuracy class
+// as the reference's per-body test (src/traversal.cpp:78), but measured on
+// the bodies actually present instead of the cubic cell.  The result of a
+// group depends only on the group, never on which worker walks it or when:
+// every schedule gives identical bits.
 //
 // Traversal keeps a stack of node ranges in LDS (children of a node occupy
 // consecutive slots); each round pops up to 64 ranges, one per lane, and the
@@ -26,8 +30,9 @@
 // drops) the self term.
 //
 // Heavy bodies (the central mass and planets: the heaviest bodies, up to 16)
-// are not in the light moments; every body adds them by an FP64 direct sum in
-// rank order.  The sum of the two parts is FP64.
+// are not in the light moments; every light body adds them by an FP64 direct
+// sum in rank order, and the heavy bodies themselves get an FP64 direct sum
+// over all bodies (k_heavy_direct).  The sum of the two parts is FP64.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -166,8 +171,19 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, TreeBuffers tb, M
     }
     nd.gm = static_cast<float>(G * m);
     nd.pad = 0;
+    // opening size: the farthest corner of the light bodies' box seen from the
+    // centre of mass (0 for an empty node)
+    double s2 = 0.0;
+    if (m != 0.0) {
+        const double* bx = tb.lbox + 6 * t;
+        const double c3[3] = {nd.x, nd.y, nd.z};
+        for (int a = 0; a < 3; ++a) {
+            const double e = fmax(c3[a] - bx[a], bx[3 + a] - c3[a]);
+            s2 += e * e;
+        }
+    }
+    nd.size2 = s2;
     if (t < M) {
-        nd.side2 = 0.0;
         nd.first = tb.leaf_first[t];
         nd.info = (tb.leaf_first[t + 1] - tb.leaf_first[t]) | kMLeaf;
     } else {
@@ -176,7 +192,6 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, TreeBuffers tb, M
         const int32_t* ch = tb.child + 8 * static_cast<size_t>(bi);
 #pragma unroll
         for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
-        nd.side2 = tb.sc->side2[tb.bdepth[bi]];
         nd.first = 1u + mb.base[bi];
         nd.info = c;
     }
@@ -390,7 +405,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                                 const double ey = fmax(fabs(nd.y - cy) - hy, 0.0);
                                 const double ez = fmax(fabs(nd.z - cz) - hz, 0.0);
                                 const double d2 = ex * ex + ey * ey + ez * ez;
-                                if (nd.side2 < theta2 * d2) {
+                                if (nd.size2 < theta2 * d2) {
                                     app = true;
                                     q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
                                                     static_cast<float>(nd.z - cz), nd.gm);
@@ -460,7 +475,8 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
             hay += s * dy;
             haz += s * dz;
         }
-        if (valid) {
+        // heavy bodies take their accelerations from k_heavy_direct instead
+        if (valid && b.hrank[i] < 0) {
             const double ax = hax + (static_cast<double>(acc.sx) + static_cast<double>(acc.cx));
             const double ay = hay + (static_cast<double>(acc.sy) + static_cast<double>(acc.cy));
             const double az = haz + (static_cast<double>(acc.sz) + static_cast<double>(acc.cz));
@@ -472,13 +488,76 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                 b.vy[i] = b.vy[i] + p.half_dt * ay;
                 b.vz[i] = b.vz[i] + p.half_dt * az;
             }
-            b.pcost[i] = n_inter;
         }
+        if (valid) b.pcost[i] = n_inter;
         if (lane == 0) {
             mb.g_inter[g] = n_inter;
             mb.g_visit[g] = n_visit;
         }
     }
+}
+
+// Accelerations of the heavy bodies: an FP64 direct sum over every other
+// body (Neumaier-compensated per thread, partials combined in thread order).
+// The heavy bodies are few and their net accelerations are small next to the
+// forces that cancel in them, so no tree truncation is taken for them.
+struct Neumaier {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x)
+    {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// Block (k, c) sums heavy body k over the c-th of kHeavyChunks body ranges;
+// k_heavy_final adds the chunk partials in chunk order.
+__global__ __launch_bounds__(kNT) void k_heavy_partial(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
+                                                       MixedParams p)
+{
+    const int k = blockIdx.x / kHeavyChunks, c = blockIdx.x % kHeavyChunks;
+    const uint32_t h = tb.heavy_slot[k];
+    const double x = b.px[h], y = b.py[h], z = b.pz[h];
+    const size_t j0 = n * c / kHeavyChunks, j1 = n * (c + 1) / kHeavyChunks;
+    Neumaier ax, ay, az;
+    for (size_t j = j0 + threadIdx.x; j < j1; j += kNT) {
+        if (j == h) continue;
+        const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+        const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
+        if (!(r2 > 0.0)) continue;
+        const double invR = 1.0 / sqrt(r2);
+        const double s = p.G * b.m[j] * (invR * invR * invR);
+        ax.add(s * dx);
+        ay.add(s * dy);
+        az.add(s * dz);
+    }
+    __shared__ double part[3][kNT];
+    part[0][threadIdx.x] = ax.value();
+    part[1][threadIdx.x] = ay.value();
+    part[2][threadIdx.x] = az.value();
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        Neumaier t;
+        for (int q = 0; q < kNT; ++q) t.add(part[threadIdx.x][q]);
+        mb.hpart[3 * static_cast<size_t>(blockIdx.x) + threadIdx.x] = t.value();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_heavy_final(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
+{
+    const int k = blockIdx.x;
+    if (threadIdx.x >= 3) return;
+    const uint32_t h = tb.heavy_slot[k];
+    Neumaier t;
+    for (int c = 0; c < kHeavyChunks; ++c) t.add(mb.hpart[3 * (static_cast<size_t>(k) * kHeavyChunks + c) + threadIdx.x]);
+    const double a = t.value();
+    double* acc = threadIdx.x == 0 ? b.ax : (threadIdx.x == 1 ? b.ay : b.az);
+    double* vel = threadIdx.x == 0 ? b.vx : (threadIdx.x == 1 ? b.vy : b.vz);
+    acc[h] = a;
+    if (p.kick) vel[h] = vel[h] + p.half_dt * a;
 }
 
 // ------------------------------------------------------------ schedule
@@ -497,21 +576,33 @@ __device__ inline uint32_t cost_bucket(float c)
     return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
 }
 
+constexpr unsigned kSchedBlocks = 256;
+
+// Source 0: one wavefront per group takes the largest previous list length of
+// its bodies.  Then every block counts its groups per bucket in LDS and adds
+// the counts to the global histogram once.
 __global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, int source)
 {
-    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    __shared__ uint32_t hist[64];
     const uint32_t G = tb.sc->n_groups;
-    if (g >= G) return;
-    float c;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
     if (source == 0) {
-        uint32_t m = 0;
-        for (uint32_t i = mb.gfirst[g]; i < mb.gfirst[g + 1]; ++i) m = max(m, b.pcost[i]);
-        c = static_cast<float>(m);
-        mb.gcost[g] = c;
-    } else {
-        c = mb.gcost[g];
+        const uint32_t waves = gridDim.x * kWaves;
+        for (uint32_t g = blockIdx.x * kWaves + (threadIdx.x >> 6); g < G; g += waves) {
+            const uint32_t i0 = mb.gfirst[g], cnt = mb.gfirst[g + 1] - i0;
+            uint32_t m = static_cast<uint32_t>(lane) < cnt ? b.pcost[i0 + lane] : 0u;
+            for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 64)));
+            if (lane == 0) mb.gcost[g] = static_cast<float>(m);
+        }
+        // the histogram below reads gcost of groups written by other blocks
+        return;
     }
-    atomicAdd(&mb.sched_hist[cost_bucket(c)], 1u);
+    __syncthreads();
+    for (uint32_t g = blockIdx.x * kNT + threadIdx.x; g < G; g += gridDim.x * kNT)
+        atomicAdd(&hist[cost_bucket(mb.gcost[g])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&mb.sched_hist[threadIdx.x], hist[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(64) void k_sched_scan(MixedBuffers mb)
@@ -524,12 +615,28 @@ __global__ __launch_bounds__(64) void k_sched_scan(MixedBuffers mb)
     mb.sched_hist[k] = 0;
 }
 
+// Every block places a tile of 256 groups: LDS counters rank the tile's groups
+// inside their buckets, one global atomic per bucket reserves the tile's range.
 __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb)
 {
-    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (g >= tb.sc->n_groups) return;
-    const uint32_t pos = atomicAdd(&mb.sched_cursor[cost_bucket(mb.gcost[g])], 1u);
-    mb.order[pos] = static_cast<uint32_t>(g);
+    __shared__ uint32_t cnt[64];
+    __shared__ uint32_t base[64];
+    const uint32_t G = tb.sc->n_groups;
+    for (uint32_t t0 = blockIdx.x * kNT; t0 < G; t0 += gridDim.x * kNT) {
+        if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t g = t0 + threadIdx.x;
+        uint32_t bk = 0, local = 0;
+        if (g < G) {
+            bk = cost_bucket(mb.gcost[g]);
+            local = atomicAdd(&cnt[bk], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64 && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&mb.sched_cursor[threadIdx.x], cnt[threadIdx.x]);
+        __syncthreads();
+        if (g < G) mb.order[base[bk] + local] = g;
+        __syncthreads();
+    }
 }
 
 inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kNT > 0 ? (n + kNT - 1) / kNT : 1); }
@@ -561,9 +668,10 @@ unsigned mixed_walk_blocks(int device)
 void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
                      hipStream_t s)
 {
-    k_sched_cost<<<nblk(n), kNT, 0, s>>>(n, b, tb, mb, source);
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0);   // previous list lengths
+    k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 1);                    // bucket histogram
     k_sched_scan<<<1, 64, 0, s>>>(mb);
-    k_sched_scatter<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+    k_sched_scatter<<<kSchedBlocks, kNT, 0, s>>>(n, tb, mb);
 }
 
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
@@ -571,6 +679,10 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
 {
     if (n == 0 || blocks == 0) return;
     k_walk_mixed<<<blocks, kNT, 0, s>>>(b, n, tb, mb, p);
+    if (p.n_heavy > 0) {
+        k_heavy_partial<<<static_cast<unsigned>(p.n_heavy * kHeavyChunks), kNT, 0, s>>>(b, n, tb, mb, p);
+        k_heavy_final<<<static_cast<unsigned>(p.n_heavy), 64, 0, s>>>(b, tb, mb, p);
+    }
 }
 
 }  // namespace bh

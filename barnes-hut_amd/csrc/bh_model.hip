@@ -1,0 +1,350 @@
+// bh_model.hip -- the learned cost model of the LEARNED form: a small MLP
+// that predicts the interaction-list length of every walk group, trained on
+// the device from the lengths the walk itself records.
+//
+// Network: 8 FP32 group features -> 32 -> 32 -> 1, ReLU on both hidden
+// layers, weights and biases stored in BF16 (round to nearest even from FP32
+// master copies), every product accumulated in FP32.  Output y is the scaled
+// log cost log2(list length + 1) / 16, formed as feature 5 (the same quantity
+// from the previous step) plus the network's output, so an untrained network
+// starts from the previous-step estimate and learns corrections to it.  Loss:
+// mean squared error on y.
+// Training: Adam on the FP32 masters, BF16 weights refreshed after each
+// update; mini-batches drawn from a device ring buffer of (features, target)
+// samples filled after every walk of the training phase.
+//
+// Parameter layout (kModelParams floats):
+//   W1[32][8] | b1[32] | W2[32][32] | b2[32] | W3[32] | b3
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bh_internal.hpp"
+
+namespace bh {
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kF = kModelFeatures;   // 8
+constexpr int kH = 32;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kF, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH,
+              kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH;
+static_assert(kOffB3 + 1 == kModelParams, "parameter layout");
+
+__device__ inline uint16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);   // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+__device__ inline float from_bf16(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+__device__ inline float relu(float v) { return v > 0.f ? v : 0.f; }
+
+// Loads the BF16 parameters of the model into LDS as FP32.
+__device__ inline void load_params(const uint16_t* __restrict__ wb, float* s)
+{
+    for (int k = threadIdx.x; k < kModelParams; k += blockDim.x) s[k] = from_bf16(wb[k]);
+    __syncthreads();
+}
+
+struct Act {
+    float z1[kH], z2[kH];
+};
+
+__device__ inline float forward(const float* __restrict__ P, const float* x, Act& a)
+{
+    for (int j = 0; j < kH; ++j) {
+        float s = P[kOffB1 + j];
+#pragma unroll
+        for (int f = 0; f < kF; ++f) s = fmaf(P[kOffW1 + j * kF + f], x[f], s);
+        a.z1[j] = s;
+    }
+    for (int j = 0; j < kH; ++j) {
+        float s = P[kOffB2 + j];
+        for (int k = 0; k < kH; ++k) s = fmaf(P[kOffW2 + j * kH + k], relu(a.z1[k]), s);
+        a.z2[j] = s;
+    }
+    // residual on feature 5 (the previous step's scaled list length): the
+    // network learns the change from the fallback estimate
+    float y = P[kOffB3];
+    for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(a.z2[j]), y);
+    return x[5] + y;
+}
+
+// ------------------------------------------------------------ features
+__device__ inline float flog2(float v) { return __log2f(v); }
+
+template <class T, class Op>
+__device__ inline T wave_reduce(T v, Op op)
+{
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One wavefront per group (grid-stride): lanes hold the group's bodies.
+__global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
+                                                        float* __restrict__ feat)
+{
+    const uint32_t G = tb.sc->n_groups;
+    const int lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kNT / 64);
+    auto dmin = [](double a, double c) { return c < a ? c : a; };
+    auto dmax = [](double a, double c) { return a < c ? c : a; };
+    auto dsum = [](double a, double c) { return a + c; };
+    auto umax = [](int a, int c) { return static_cast<int>(max(static_cast<uint32_t>(a), static_cast<uint32_t>(c))); };
+    auto umin = [](int a, int c) { return static_cast<int>(min(static_cast<uint32_t>(a), static_cast<uint32_t>(c))); };
+    for (uint32_t g = blockIdx.x * (kNT / 64) + (threadIdx.x >> 6); g < G; g += waves) {
+        const uint32_t i0 = mb.gfirst[g], cnt = mb.gfirst[g + 1] - i0;
+        const bool v = static_cast<uint32_t>(lane) < cnt;
+        const uint32_t i = i0 + (v ? static_cast<uint32_t>(lane) : 0u);
+        const double x = b.px[i], y = b.py[i], z = b.pz[i];
+        const double lx = wave_reduce(x, dmin), ly = wave_reduce(y, dmin), lz = wave_reduce(z, dmin);
+        const double hx = wave_reduce(x, dmax), hy = wave_reduce(y, dmax), hz = wave_reduce(z, dmax);
+        const double mass = wave_reduce((v && b.hrank[i] < 0) ? b.m[i] : 0.0, dsum);
+        const uint32_t pc = b.pcost[i];
+        const uint32_t cmax = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0u), umax));
+        const uint32_t cmin = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0xffffffffu), umin));
+        if (lane != 0) continue;
+        const double ex = hx - lx, ey = hy - ly, ez = hz - lz;
+        const double ext = fmax(ex, fmax(ey, ez));
+        const double cx = 0.5 * (lx + hx), cy = 0.5 * (ly + hy), cz = 0.5 * (lz + hz);
+        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+        float* f = feat + kF * static_cast<size_t>(g);
+        f[0] = static_cast<float>(mb.gdepth[i0]) / 21.f;
+        f[1] = static_cast<float>(cnt) / 64.f;
+        f[2] = flog2(static_cast<float>(ext) + 1e-12f) / 16.f;
+        f[3] = flog2(static_cast<float>(dist) + 1e-12f) / 8.f;
+        f[4] = (log10f(static_cast<float>(mass) + 1.f) - 20.f) / 5.f;
+        f[5] = flog2(static_cast<float>(cmax) + 1.f) / 16.f;
+        f[6] = flog2(static_cast<float>(cmin) + 1.f) / 16.f;
+        f[7] = flog2(static_cast<float>(ez / (ext + 1e-30)) + 1e-6f) / 8.f;
+    }
+}
+
+// ------------------------------------------------------------ inference
+// gcost = 2^(16 y) - 1, the predicted list length of each group.
+__global__ __launch_bounds__(kNT) void k_model_cost(const float* __restrict__ feat, const uint16_t* __restrict__ wb,
+                                                    TreeBuffers tb, MixedBuffers mb)
+{
+    __shared__ float P[kModelParams];
+    load_params(wb, P);
+    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (g >= tb.sc->n_groups) return;
+    float x[kF];
+#pragma unroll
+    for (int f = 0; f < kF; ++f) x[f] = feat[kF * g + f];
+    Act a;
+    const float y = forward(P, x, a);
+    mb.gcost[g] = fmaxf(exp2f(16.f * fminf(y, 1.5f)) - 1.f, 0.f);
+}
+
+__global__ __launch_bounds__(kNT) void k_model_forward(const float* __restrict__ feat, size_t m,
+                                                       const uint16_t* __restrict__ wb, float* __restrict__ out)
+{
+    __shared__ float P[kModelParams];
+    load_params(wb, P);
+    const size_t s = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (s >= m) return;
+    float x[kF];
+#pragma unroll
+    for (int f = 0; f < kF; ++f) x[f] = feat[kF * s + f];
+    Act a;
+    out[s] = forward(P, x, a);
+}
+
+// ------------------------------------------------------------- samples
+// Appends the groups of this step to the ring buffer: features and target
+// log2(list length + 1) / 16.
+__global__ __launch_bounds__(kNT) void k_append_samples(const float* __restrict__ feat, TreeBuffers tb,
+                                                        MixedBuffers mb, ModelBuffers md)
+{
+    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t G = tb.sc->n_groups;
+    if (g >= G) return;
+    const uint64_t base = *md.n_seen;
+    const uint64_t slot = (base + g) % md.capacity;
+#pragma unroll
+    for (int f = 0; f < kF; ++f) md.sfeat[kF * slot + f] = feat[kF * g + f];
+    md.starget[slot] = flog2(static_cast<float>(mb.g_inter[g]) + 1.f) / 16.f;
+}
+
+__global__ void k_samples_commit(TreeBuffers tb, ModelBuffers md) { *md.n_seen += tb.sc->n_groups; }
+
+// ------------------------------------------------------------- training
+__device__ inline uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+constexpr int kChunk = 64;    // samples per chunk held in LDS
+
+// One block.  Gradients of the mean squared error over `batch` samples,
+// drawn (with a hash of iter) from the buffer or taken in order when
+// sample_mode == 0; then, if update, one Adam step on the FP32 masters and a
+// refresh of the BF16 weights.  grad_out / loss_out (optional) receive the
+// gradient and loss.
+__global__ __launch_bounds__(kNT) void k_model_train(ModelBuffers md, const float* __restrict__ xin,
+                                                     const float* __restrict__ tin, uint64_t m_in, int batch,
+                                                     int sample_mode, uint32_t iter, float lr, int update,
+                                                     float* __restrict__ grad_out, float* __restrict__ loss_out)
+{
+    __shared__ float P[kModelParams];
+    __shared__ float sx[kChunk][kF];
+    __shared__ float sh1[kChunk][kH];
+    __shared__ float sd2[kChunk][kH];
+    __shared__ float sd1[kChunk][kH];
+    __shared__ float sh2[kChunk][kH];
+    __shared__ float sdy[kChunk];
+    __shared__ float sloss[kNT];
+    load_params(md.wbf, P);
+    const int t = threadIdx.x;
+    const float* X = xin ? xin : md.sfeat;
+    const float* T = tin ? tin : md.starget;
+    const uint64_t avail = xin ? m_in : min<uint64_t>(*md.n_seen, md.capacity);
+    float gacc[6] = {0, 0, 0, 0, 0, 0};   // params t, t+256, ... (<= 6 per thread)
+    float lsum = 0.f;
+    const float inv_b = 1.f / static_cast<float>(batch);
+    for (int c0 = 0; c0 < batch; c0 += kChunk) {
+        const int cn = min(kChunk, batch - c0);
+        if (t < cn && avail > 0) {
+            const int k = c0 + t;
+            const uint64_t s = sample_mode == 0 ? static_cast<uint64_t>(k) % avail
+                                                : static_cast<uint64_t>(mix32(iter * 0x9e3779b9u + k * 0x85ebca6bu + 1u)) % avail;
+            float x[kF];
+#pragma unroll
+            for (int f = 0; f < kF; ++f) {
+                x[f] = X[kF * s + f];
+                sx[t][f] = x[f];
+            }
+            Act a;
+            const float y = forward(P, x, a);
+            const float d = y - T[s];
+            lsum += d * d;
+            const float dy = 2.f * d * inv_b;
+            sdy[t] = dy;
+            float d2[kH];
+            for (int j = 0; j < kH; ++j) {
+                sh2[t][j] = relu(a.z2[j]);
+                sh1[t][j] = relu(a.z1[j]);
+                d2[j] = a.z2[j] > 0.f ? dy * P[kOffW3 + j] : 0.f;
+                sd2[t][j] = d2[j];
+            }
+            for (int k2 = 0; k2 < kH; ++k2) {
+                float s1 = 0.f;
+                for (int j = 0; j < kH; ++j) s1 = fmaf(d2[j], P[kOffW2 + j * kH + k2], s1);
+                sd1[t][k2] = a.z1[k2] > 0.f ? s1 : 0.f;
+            }
+        } else if (t < kChunk) {
+            sdy[t] = 0.f;
+            for (int j = 0; j < kH; ++j) sh2[t][j] = sh1[t][j] = sd2[t][j] = sd1[t][j] = 0.f;
+            for (int f = 0; f < kF; ++f) sx[t][f] = 0.f;
+        }
+        __syncthreads();
+        for (int q = 0; q < 6; ++q) {
+            const int pidx = t + q * kNT;
+            if (pidx >= kModelParams) break;
+            float acc = 0.f;
+            if (pidx < kOffB1) {
+                const int j = pidx / kF, f = pidx % kF;
+                for (int s = 0; s < cn; ++s) acc = fmaf(sd1[s][j], sx[s][f], acc);
+            } else if (pidx < kOffW2) {
+                const int j = pidx - kOffB1;
+                for (int s = 0; s < cn; ++s) acc += sd1[s][j];
+            } else if (pidx < kOffB2) {
+                const int j = (pidx - kOffW2) / kH, k2 = (pidx - kOffW2) % kH;
+                for (int s = 0; s < cn; ++s) acc = fmaf(sd2[s][j], sh1[s][k2], acc);
+            } else if (pidx < kOffW3) {
+                const int j = pidx - kOffB2;
+                for (int s = 0; s < cn; ++s) acc += sd2[s][j];
+            } else if (pidx < kOffB3) {
+                const int j = pidx - kOffW3;
+                for (int s = 0; s < cn; ++s) acc = fmaf(sdy[s], sh2[s][j], acc);
+            } else {
+                for (int s = 0; s < cn; ++s) acc += sdy[s];
+            }
+            gacc[q] += acc;
+        }
+        __syncthreads();
+    }
+    sloss[t] = lsum;
+    __syncthreads();
+    if (t == 0) {
+        float l = 0.f;
+        for (int k = 0; k < kNT; ++k) l += sloss[k];
+        l *= inv_b;
+        if (loss_out) *loss_out = l;
+        *md.last_loss = l;
+    }
+    for (int q = 0; q < 6; ++q) {
+        const int pidx = t + q * kNT;
+        if (pidx >= kModelParams) break;
+        const float g = gacc[q];
+        if (grad_out) grad_out[pidx] = g;
+        if (update) {
+            // Adam (beta1 0.9, beta2 0.999, eps 1e-8) with bias correction
+            const float b1 = 0.9f, b2 = 0.999f;
+            const float mm = b1 * md.adam_m[pidx] + (1.f - b1) * g;
+            const float vv = b2 * md.adam_v[pidx] + (1.f - b2) * g * g;
+            md.adam_m[pidx] = mm;
+            md.adam_v[pidx] = vv;
+            const float step = static_cast<float>(iter + 1);
+            const float mh = mm / (1.f - powf(b1, step));
+            const float vh = vv / (1.f - powf(b2, step));
+            const float w = md.wf32[pidx] - lr * mh / (sqrtf(vh) + 1e-8f);
+            md.wf32[pidx] = w;
+            md.wbf[pidx] = to_bf16(w);
+        }
+    }
+}
+
+__global__ void k_refresh_bf16(ModelBuffers md)
+{
+    for (int k = threadIdx.x; k < kModelParams; k += blockDim.x) md.wbf[k] = to_bf16(md.wf32[k]);
+}
+
+inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kNT > 0 ? (n + kNT - 1) / kNT : 1); }
+
+}  // namespace
+
+void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
+                           hipStream_t s)
+{
+    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat);
+}
+
+void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
+                       const MixedBuffers& mb, hipStream_t s)
+{
+    k_model_cost<<<nblk(n), kNT, 0, s>>>(feat, md.wbf, tb, mb);
+}
+
+void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s)
+{
+    if (m == 0) return;
+    k_model_forward<<<nblk(m), kNT, 0, s>>>(feat, m, md.wbf, out);
+}
+
+void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, const MixedBuffers& mb,
+                           const ModelBuffers& md, hipStream_t s)
+{
+    k_append_samples<<<nblk(n), kNT, 0, s>>>(feat, tb, mb, md);
+    k_samples_commit<<<1, 1, 0, s>>>(tb, md);
+}
+
+void launch_model_train(const ModelBuffers& md, const float* x, const float* t, uint64_t m, int batch,
+                        int sample_mode, uint32_t iter, float lr, bool update, float* grad_out, float* loss_out,
+                        hipStream_t s)
+{
+    k_model_train<<<1, kNT, 0, s>>>(md, x, t, m, batch, sample_mode, iter, lr, update ? 1 : 0, grad_out, loss_out);
+}
+
+void launch_refresh_bf16(const ModelBuffers& md, hipStream_t s) { k_refresh_bf16<<<1, kNT, 0, s>>>(md); }
+
+}  // namespace bh

@@ -297,6 +297,7 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
         if (tb.lraw) {
             // moments without the heavy bodies, which the mixed walk sums directly
             double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+            double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
             for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
                 if (b.hrank[i] >= 0) continue;
                 const double mi = b.m[i];
@@ -304,6 +305,16 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
                 lx += b.px[i] * mi;
                 ly += b.py[i] * mi;
                 lz += b.pz[i] * mi;
+                const double xi[3] = {b.px[i], b.py[i], b.pz[i]};
+                for (int a = 0; a < 3; ++a) {
+                    bl[a] = dmin(bl[a], xi[a]);
+                    bu[a] = dmax(bu[a], xi[a]);
+                }
+            }
+            double* bx = tb.lbox + 6 * j;
+            for (int a = 0; a < 3; ++a) {
+                bx[a] = bl[a];
+                bx[3 + a] = bu[a];
             }
             double* q = tb.lraw + 4 * j;
             q[0] = lm;
@@ -402,6 +413,7 @@ __global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb,
     r[3] = sz;
     if (tb.lraw) {
         double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+        double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int32_t ck = c[k];
@@ -411,6 +423,16 @@ __global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb,
             lx += q[1];
             ly += q[2];
             lz += q[3];
+            const double* cb = tb.lbox + 6 * static_cast<size_t>(ck);
+            for (int a = 0; a < 3; ++a) {
+                bl[a] = dmin(bl[a], cb[a]);
+                bu[a] = dmax(bu[a], cb[3 + a]);
+            }
+        }
+        double* bx = tb.lbox + 6 * (static_cast<size_t>(M) + bi);
+        for (int a = 0; a < 3; ++a) {
+            bx[a] = bl[a];
+            bx[3 + a] = bu[a];
         }
         double* q = tb.lraw + 4 * (static_cast<size_t>(M) + bi);
         q[0] = lm;

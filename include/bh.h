@@ -153,6 +153,46 @@ int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
 #define BH_SCHED_MODEL 2
 int bh_set_schedule(bh_ctx *ctx, int mode);
 
+/* ------------------------------------------------ learned cost model ---
+ * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
+ * accumulation) predicts log2(list length + 1) / 16 of every walk group as
+ * feature 5 (the previous step's value) plus its output; the
+ * LEARNED form serves groups heaviest-first by its predictions.  It trains on
+ * the device from the list lengths the walk records, during the first
+ * train_steps LEARNED evaluations (iters_per_step Adam updates of `batch`
+ * samples each), and is used from then on.  Features per group:
+ *   [0] tree depth / 21         [1] bodies / 64
+ *   [2] log2(box extent) / 16   [3] log2(distance of box centre to origin) / 8
+ *   [4] (log10(light mass + 1) - 20) / 5
+ *   [5] log2(max list length of its bodies last step + 1) / 16
+ *   [6] log2(min list length of its bodies last step + 1) / 16
+ *   [7] log2(z extent / box extent) / 8 */
+#define BH_MODEL_FEATURES 8
+#define BH_MODEL_PARAMS 1377
+/* Defaults: 50 steps, 16 updates per step, batch 2048, learning rate 3e-3. */
+int bh_model_configure(bh_ctx *ctx, int train_steps, int iters_per_step, int batch, float lr);
+/* Fresh weights from `seed` (He-uniform), empty sample buffer, not ready. */
+int bh_model_reset(bh_ctx *ctx, uint64_t seed);
+/* FP32 master weights, layout W1[32][8] b1[32] W2[32][32] b2[32] W3[32] b3. */
+int bh_model_get_params(bh_ctx *ctx, float *params);
+/* Sets the masters (the BF16 copy follows); clears the Adam state. */
+int bh_model_set_params(bh_ctx *ctx, const float *params);
+/* Model output y for m feature rows (m x 8, row-major). */
+int bh_model_forward(bh_ctx *ctx, int64_t m, const float *features, float *out);
+/* Gradient of the mean squared error (1/m) sum (y - t)^2 over m rows with
+ * respect to every parameter, evaluated with the BF16 weights; loss may be
+ * NULL. */
+int bh_model_gradients(bh_ctx *ctx, int64_t m, const float *features, const float *targets, float *grad,
+                       float *loss);
+/* iters Adam updates on mini-batches of `batch` rows drawn from the given
+ * rows; marks the model ready.  final_loss may be NULL. */
+int bh_model_train(bh_ctx *ctx, int64_t m, const float *features, const float *targets, int iters, int batch,
+                   float lr, float *final_loss);
+/* ready: model in use; samples: collected; updates: Adam steps so far. */
+int bh_model_status(bh_ctx *ctx, int *ready, int64_t *samples, int64_t *updates, float *last_loss);
+/* Features of the groups of the last LEARNED evaluation (G x 8). */
+int bh_get_group_features(bh_ctx *ctx, int64_t capacity, float *features);
+
 /* Walk groups of the last MIXED / LEARNED evaluation: up to 64
  * Morton-consecutive bodies walked together.  *n_groups gets the count; when
  * capacity >= count, first_body[g] (count + 1 entries, last = n),

@@ -1,0 +1,132 @@
+"""GPU tests of the learned schedule: HIP forward / gradient kernels against
+the numpy reference (tests/mlp_ref.py, itself pinned by finite differences),
+training on the device, forces bit-identical to the MIXED schedule, and what
+the trained network gains over the schedules without it."""
+import json
+import pathlib
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+HOUR = 1.0 / 24.0
+OUT = pathlib.Path(__file__).resolve().parents[1] / "gpurun_out"
+
+
+def say(*a):
+    print("[gpu-case]", *a, file=sys.stderr, flush=True)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def test_model_forward_and_gradient_match_reference(bh):
+    rng = np.random.default_rng(2)
+    with bh.Engine() as eng:
+        p = mlp_ref.init(7)
+        eng.model_set_params(p)
+        assert np.array_equal(eng.model_get_params(), p)
+        x = rng.normal(size=(777, 8)).astype(np.float32)
+        t = (0.2 * rng.normal(size=777)).astype(np.float32)
+        y = eng.model_forward(x)
+        yr, _ = mlp_ref.forward(p, x)
+        assert np.allclose(y, yr, rtol=1e-5, atol=1e-6)
+        g, loss = eng.model_gradients(x, t)
+        lr, gr = mlp_ref.loss_and_grad(p, x, t)
+        assert abs(loss - lr) <= 1e-5 * max(lr, 1e-6)
+        scale = np.abs(gr).max()
+        assert np.allclose(g, gr, rtol=1e-4, atol=1e-5 * scale), np.abs(g - gr).max() / scale
+
+
+def test_model_training_reduces_loss(bh):
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(8192, 8)).astype(np.float32)
+    t = (0.5 + 0.3 * np.tanh(x[:, 5] + 0.5 * x[:, 2]) - 0.1 * x[:, 1]).astype(np.float32)
+    with bh.Engine() as eng:
+        _, l0 = eng.model_gradients(x[:4096], t[:4096])
+        lf = eng.model_train(x, t, iters=400, batch=1024, lr=3e-3)
+        _, l1 = eng.model_gradients(x[:4096], t[:4096])
+        say("loss", l0, "->", l1)
+        assert l1 < 0.1 * l0
+
+
+def _trojan(bh, n, seed=3):
+    return bh.generate(bh.GEN_TROJAN, seed, n)
+
+
+def test_learned_forces_bit_identical_to_mixed(bh):
+    pos, vel, m = _trojan(bh, 60000)
+    res = {}
+    for name, form, sched in (("mixed", bh.FORM_MIXED, bh.SCHED_KEY_ORDER),
+                              ("mixed_prev", bh.FORM_MIXED, bh.SCHED_PREV_COST),
+                              ("learned", bh.FORM_LEARNED, bh.SCHED_MODEL)):
+        say("schedule", name)
+        with bh.Engine(form=form, theta=1.05) as eng:
+            eng.set_schedule(sched)
+            eng.model_configure(train_steps=4, iters_per_step=8, batch=1024, lr=3e-3)
+            eng.set_bodies(pos, vel, m)
+            eng.step(7, HOUR)   # spans training (4 steps) and model use (3 steps)
+            res[name] = eng.get_bodies()
+            if form == bh.FORM_LEARNED:
+                ready, samples, updates, loss = eng.model_status()
+                assert ready and samples > 0 and updates == 32
+    for name in ("mixed_prev", "learned"):
+        for k in ("pos", "vel", "acc"):
+            assert np.array_equal(bits(res[name][k]), bits(res["mixed"][k])), (name, k)
+        assert np.array_equal(res[name]["ids"], res["mixed"]["ids"])
+
+
+def test_learned_gain_over_schedules_without_it(bh):
+    """Trains the model on 50 steps of a Trojan-rich system (strongly uneven
+    walk cost), then measures, on the following steps, the prediction error of
+    the network against the previous-step counts and the walk time of the
+    three schedules.  The figures go to gpurun_out/learned_gain.json."""
+    n = 262144
+    pos, vel, m = _trojan(bh, n, seed=5)
+    out = {}
+    with bh.Engine(form=bh.FORM_LEARNED, theta=1.05) as eng:
+        eng.set_bodies(pos, vel, m)
+        eng.step(50, HOUR)
+        ready, samples, updates, loss = eng.model_status()
+        assert ready
+        say("trained: samples", samples, "updates", updates, "loss", loss)
+        # prediction quality on the next step: network vs previous counts
+        eng.step(1, HOUR)
+        feat = eng.group_features()
+        first, inter, visit = eng.groups()
+        pred = 2.0 ** (16 * eng.model_forward(feat)) - 1
+        prev = 2.0 ** (16 * feat[:, 5]) - 1
+        tgt = inter.astype(np.float64)
+        e_net = float(np.mean(np.abs(np.log2(pred + 1) - np.log2(tgt + 1))))
+        e_prev = float(np.mean(np.abs(np.log2(prev + 1) - np.log2(tgt + 1))))
+        e_const = float(np.mean(np.abs(np.log2(tgt + 1) - np.mean(np.log2(tgt + 1)))))
+        out.update(mean_abs_log2_err_network=e_net, mean_abs_log2_err_prev_counts=e_prev,
+                   mean_abs_log2_err_constant=e_const, groups=int(len(inter)), train_loss=float(loss))
+        say("log2 error: network", e_net, "prev", e_prev, "constant", e_const)
+        assert e_net < e_const
+        state = eng.get_bodies()
+    # walk time of each schedule from the same state
+    for name, form, sched in (("key_order", bh.FORM_MIXED, bh.SCHED_KEY_ORDER),
+                              ("prev_counts", bh.FORM_MIXED, bh.SCHED_PREV_COST)):
+        with bh.Engine(form=form, theta=1.05) as e2:
+            e2.set_schedule(sched)
+            e2.set_bodies(state["pos"], state["vel"], state["mass"], state["ids"])
+            e2.step(3, HOUR)
+            e2.step(10, HOUR)
+            ph, k = e2.step_phase_ms()
+            out[f"walk_ms_{name}"] = float(ph[3])
+    with bh.Engine(form=bh.FORM_LEARNED, theta=1.05) as e3:
+        e3.set_bodies(pos, vel, m)
+        e3.step(50, HOUR)
+        e3.step(10, HOUR)
+        ph, k = e3.step_phase_ms()
+        out["walk_ms_network"] = float(ph[3])
+    say("walk ms", {k: v for k, v in out.items() if k.startswith("walk")})
+    OUT.mkdir(exist_ok=True)
+    (OUT / "learned_gain.json").write_text(json.dumps(out, indent=1))
+    assert out["walk_ms_network"] < 1.10 * out["walk_ms_key_order"]
