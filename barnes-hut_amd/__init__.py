@@ -77,6 +77,7 @@ SIGNATURES = [
     ("bh_direct_sum", C.c_int, [_P, _I64, _P, _D, _D, _P]),
     ("bh_get_groups", C.c_int, [_P, C.POINTER(_I64), _I64, _P, _P, _P]),
     ("bh_set_schedule", C.c_int, [_P, C.c_int]),
+    ("bh_get_group_timeline", C.c_int, [_P, _I64, _P, _P, _P]),
     ("bh_model_configure", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_float]),
     ("bh_model_reset", C.c_int, [_P, C.c_uint64]),
     ("bh_model_get_params", C.c_int, [_P, _P]),
@@ -345,6 +346,16 @@ class Engine:
         out = np.zeros((k.value, 8), np.float32)
         self._ck(self.lib.bh_get_group_features(self.h, k.value, _ptr(out)))
         return out
+
+    def group_timeline(self):
+        """(t_start, t_end) in 100 MHz device ticks and worker per group."""
+        k = _I64(0)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
+        t0 = np.zeros(k.value, np.uint64)
+        t1 = np.zeros(k.value, np.uint64)
+        w = np.zeros(k.value, np.uint32)
+        self._ck(self.lib.bh_get_group_timeline(self.h, k.value, _ptr(t0), _ptr(t1), _ptr(w)))
+        return t0, t1, w
 
     def groups(self):
         """Walk groups of the last mixed evaluation: first_body (G+1),

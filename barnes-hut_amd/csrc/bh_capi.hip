@@ -141,8 +141,8 @@ int reserve(bh_ctx* c, size_t n)
     A(t.heavy_slot, 64); A(t.gflag, N);
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
-    A(m.g_inter, N); A(m.g_visit, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
-    A(m.sched_hist, 64); A(m.sched_cursor, 64); A(m.hpart, 3 * bh::kMaxHeavy * bh::kHeavyChunks);
+    A(m.g_inter, N); A(m.g_visit, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
+    A(m.sched_hist, 64); A(m.sched_cursor, 64); A(m.hpart, 3 * bh::kMaxHeavy * bh::kHeavyChunks); A(m.dhist, 32);
     A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
     HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
@@ -707,6 +707,21 @@ int bh_get_groups(bh_ctx* c, int64_t* n_groups, int64_t capacity, uint32_t* firs
     if (first_body) HIP_TRY(c, hipMemcpy(first_body, c->mb.gfirst, (G + 1) * 4, hipMemcpyDeviceToHost));
     if (interactions) HIP_TRY(c, hipMemcpy(interactions, c->mb.g_inter, G * 4, hipMemcpyDeviceToHost));
     if (visited) HIP_TRY(c, hipMemcpy(visited, c->mb.g_visit, G * 4, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+int bh_get_group_timeline(bh_ctx* c, int64_t capacity, uint64_t* t_start, uint64_t* t_end, uint32_t* worker)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (!c->have_groups) return fail(c, BH_ERR_STATE, "no mixed evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    const size_t G = sc.n_groups;
+    if (capacity < static_cast<int64_t>(G)) return fail(c, BH_ERR_OVERFLOW, "%zu groups", G);
+    if (t_start) HIP_TRY(c, hipMemcpy(t_start, c->mb.g_t0, G * 8, hipMemcpyDeviceToHost));
+    if (t_end) HIP_TRY(c, hipMemcpy(t_end, c->mb.g_t1, G * 8, hipMemcpyDeviceToHost));
+    if (worker) HIP_TRY(c, hipMemcpy(worker, c->mb.g_worker, G * 4, hipMemcpyDeviceToHost));
     return BH_OK;
 }
 

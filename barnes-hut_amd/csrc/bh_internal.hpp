@@ -171,10 +171,14 @@ struct MixedBuffers {
     uint8_t* gdepth;    // N: tree depth of the group's cells (at the start body)
     uint32_t* g_inter;  // N: interactions evaluated by each group
     uint32_t* g_visit;  // N: nodes visited by each group
+    uint64_t* g_t0;     // N: wall clock (100 MHz) when a worker took the group
+    uint64_t* g_t1;     // N: wall clock when it finished
+    uint32_t* g_worker; // N: worker wave that walked the group
     uint32_t* order;    // N: group order of the scheduled walk
     uint32_t* chunk;    // workers + 1: static chunk starts into order[]
     float* gcost;       // N: estimated cost of each group
     double* hpart;      // heavy direct sums: kMaxHeavy x kHeavyChunks x 3 partials
+    uint32_t* dhist;    // 22: boundaries splitting at each depth
     uint32_t* sched_hist;    // 64 cost buckets (left at zero)
     uint32_t* sched_cursor;  // 64
 };

@@ -70,7 +70,44 @@ __device__ inline uint32_t unit_first_body(const TreeBuffers& tb, uint32_t M, ui
     return t < M ? tb.leaf_first[t] : tb.leaf_first[tb.blo[t - M]];
 }
 
-// group starts (flags at the first body of every group)
+// Cell occupancy by depth: hist[d] = number of boundaries u[j] | u[j+1] that
+// split at depth d, so depth d holds 1 + sum_{e < d} hist[e] non-empty cells.
+// One block; the group marks derive the group size limit from it.
+__global__ __launch_bounds__(1024) void k_split_hist(TreeBuffers tb, MixedBuffers mb)
+{
+    __shared__ uint32_t h[kMaxDepth + 1];
+    if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t M = tb.sc->n_leaves;
+    for (uint32_t j = threadIdx.x; j + 1 < M; j += blockDim.x) atomicAdd(&h[tb.delta[j]], 1u);
+    __syncthreads();
+    if (threadIdx.x <= kMaxDepth) mb.dhist[threadIdx.x] = h[threadIdx.x];
+}
+
+__device__ inline double box_extent(const double* bx)
+{
+    const double e = fmax(bx[3] - bx[0], fmax(bx[4] - bx[1], bx[5] - bx[2]));
+    return e > 0.0 ? e : 0.0;   // an empty box (no light bodies) has extent 0
+}
+
+// Largest box extent a walk group may have: twice the cell side at the first
+// depth whose non-empty cells number at least n/32 (cells there hold ~32
+// bodies on average).  A group much wider than its neighbours' spacing would
+// force the walk to open everything its box overlaps.
+__device__ inline double group_extent_limit(const TreeBuffers& tb, const MixedBuffers& mb, size_t n)
+{
+    uint32_t cells = 1;
+    int d = 0;
+    for (; d < kMaxDepth && static_cast<size_t>(cells) * 32 < n; ++d) cells += mb.dhist[d];
+    return 2.0 * sqrt(tb.sc->side2[d]);
+}
+
+// group starts (flags at the first body of every group).  A unit is "good"
+// when it holds <= 64 bodies and its light bodies' box is within the extent
+// limit (a leaf always qualifies); goodness passes to children.  Groups are
+// the maximal good units, and runs of good siblings are packed while the
+// count fits 64 and their joint box stays within the limit.  Everything here
+// depends on the bodies alone, so every schedule sees the same groups.
 __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, MixedBuffers mb)
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
@@ -79,7 +116,11 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     const uint32_t u = static_cast<uint32_t>(t);
     const uint32_t cnt = unit_count(tb, M, u);
     const uint32_t depth = u < M ? kMaxDepth : tb.bdepth[u - M];
-    if (tb.parent[u] == kNone && cnt <= kGroupMax) {
+    const double emax = group_extent_limit(tb, mb, n);
+    auto good = [&](uint32_t v, uint32_t cv) {
+        return cv <= static_cast<uint32_t>(kGroupMax) && (v < M || box_extent(tb.lbox + 6 * static_cast<size_t>(v)) <= emax);
+    };
+    if (tb.parent[u] == kNone && good(u, cnt)) {
         const uint32_t f = unit_first_body(tb, M, u);
         tb.gflag[f] = 1;
         mb.gdepth[f] = static_cast<uint8_t>(depth);
@@ -95,26 +136,45 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
         }
         return;
     }
-    if (cnt <= kGroupMax) return;
+    if (good(u, cnt)) return;   // inside a group formed above
     const int32_t* c = tb.child + 8 * static_cast<size_t>(u - M);
     uint32_t cur = 0;
     bool open = false;
+    double rl[3] = {0, 0, 0}, ru[3] = {0, 0, 0};   // joint light box of the open run
     for (int k = 0; k < 8; ++k) {
         const int32_t ck = c[k];
         if (ck < 0) continue;
-        const uint32_t cc = unit_count(tb, M, static_cast<uint32_t>(ck));
-        if (cc > kGroupMax) {
+        const uint32_t cv = static_cast<uint32_t>(ck);
+        const uint32_t cc = unit_count(tb, M, cv);
+        if (!good(cv, cc)) {
             open = false;
             continue;
         }
-        if (!open || cur + cc > kGroupMax) {
-            const uint32_t f = unit_first_body(tb, M, static_cast<uint32_t>(ck));
+        const double* bx = tb.lbox + 6 * static_cast<size_t>(cv);
+        const bool empty = !(bx[0] <= bx[3]);
+        bool fits = open && cur + cc <= static_cast<uint32_t>(kGroupMax);
+        if (fits && !empty) {
+            double e = 0.0;
+            for (int a = 0; a < 3; ++a) e = fmax(e, fmax(ru[a], bx[3 + a]) - fmin(rl[a], bx[a]));
+            fits = e <= emax;
+        }
+        if (!fits) {
+            const uint32_t f = unit_first_body(tb, M, cv);
             tb.gflag[f] = 1;
             mb.gdepth[f] = static_cast<uint8_t>(depth + 1);
             cur = 0;
             open = true;
+            for (int a = 0; a < 3; ++a) {
+                rl[a] = INFINITY;
+                ru[a] = -INFINITY;
+            }
         }
         cur += cc;
+        if (!empty)
+            for (int a = 0; a < 3; ++a) {
+                rl[a] = fmin(rl[a], bx[a]);
+                ru[a] = fmax(ru[a], bx[3 + a]);
+            }
     }
 }
 
@@ -332,6 +392,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         }
         g = uni(g);
         if (g >= G) continue;   // a schedule entry outside the groups of this step
+        const uint64_t t_start = wall_clock64();
         const uint32_t b0 = mb.gfirst[g];
         const uint32_t cnt = mb.gfirst[g + 1] - b0;
         const bool valid = static_cast<uint32_t>(lane) < cnt;
@@ -493,6 +554,9 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         if (lane == 0) {
             mb.g_inter[g] = n_inter;
             mb.g_visit[g] = n_visit;
+            mb.g_t0[g] = t_start;
+            mb.g_t1[g] = wall_clock64();
+            mb.g_worker[g] = worker;
         }
     }
 }
@@ -645,6 +709,7 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 
 void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s)
 {
+    k_split_hist<<<1, 1024, 0, s>>>(tb, mb);
     k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb);
     scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
     k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);

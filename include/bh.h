@@ -201,6 +201,11 @@ int bh_get_group_features(bh_ctx *ctx, int64_t capacity, float *features);
 int bh_get_groups(bh_ctx *ctx, int64_t *n_groups, int64_t capacity, uint32_t *first_body,
                   uint32_t *interactions, uint32_t *visited);
 
+/* Timeline of the last mixed walk: for each group the device wall clock
+ * (100 MHz ticks) when a worker wave took it and when it finished, and the
+ * worker's index.  Arrays sized by the group count of bh_get_groups. */
+int bh_get_group_timeline(bh_ctx *ctx, int64_t capacity, uint64_t *t_start, uint64_t *t_end, uint32_t *worker);
+
 /* FP64 direct sum on the device for m sample bodies (indices into the
  * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 with
  * Neumaier-compensated sums, r^2 = |d|^2 + soft2.  The reference for force
