@@ -179,6 +179,7 @@ struct MixedBuffers {
     float* gcost;       // N: estimated cost of each group
     double* hpart;      // heavy direct sums: kMaxHeavy x kHeavyChunks x 3 partials
     uint32_t* dhist;    // 22: boundaries splitting at each depth
+    uint32_t* dhist_part;  // per-block counts behind dhist
     uint32_t* sched_hist;    // 64 cost buckets (left at zero)
     uint32_t* sched_cursor;  // 64
 };

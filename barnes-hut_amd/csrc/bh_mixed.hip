@@ -72,16 +72,42 @@ __device__ inline uint32_t unit_first_body(const TreeBuffers& tb, uint32_t M, ui
 
 // Cell occupancy by depth: hist[d] = number of boundaries u[j] | u[j+1] that
 // split at depth d, so depth d holds 1 + sum_{e < d} hist[e] non-empty cells.
-// One block; the group marks derive the group size limit from it.
-__global__ __launch_bounds__(1024) void k_split_hist(TreeBuffers tb, MixedBuffers mb)
+// Each wavefront counts 64 boundaries per round with one ballot per depth
+// (no atomics in the loop); every block leaves its 22 counts in
+// mb.dhist_part and k_split_reduce adds them in block order.
+constexpr unsigned kHistBlocks = 128;
+
+__global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb, MixedBuffers mb)
 {
-    __shared__ uint32_t h[kMaxDepth + 1];
-    if (threadIdx.x <= kMaxDepth) h[threadIdx.x] = 0;
-    __syncthreads();
+    __shared__ uint32_t h[kWaves][kMaxDepth + 1];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t M = tb.sc->n_leaves;
-    for (uint32_t j = threadIdx.x; j + 1 < M; j += blockDim.x) atomicAdd(&h[tb.delta[j]], 1u);
+    uint32_t cnt[kMaxDepth + 1];
+#pragma unroll
+    for (int d = 0; d <= kMaxDepth; ++d) cnt[d] = 0;
+    const uint32_t stride = gridDim.x * kNT;
+    for (uint32_t j0 = blockIdx.x * kNT + w * 64; j0 + 1 < M; j0 += stride) {
+        const uint32_t j = j0 + static_cast<uint32_t>(lane);
+        const int d = (j + 1 < M) ? static_cast<int>(tb.delta[j]) : -1;
+#pragma unroll
+        for (int e = 0; e <= kMaxDepth; ++e) cnt[e] += static_cast<uint32_t>(__popcll(__ballot(d == e)));
+    }
+    if (lane == 0)
+        for (int d = 0; d <= kMaxDepth; ++d) h[w][d] = cnt[d];
     __syncthreads();
-    if (threadIdx.x <= kMaxDepth) mb.dhist[threadIdx.x] = h[threadIdx.x];
+    if (threadIdx.x <= kMaxDepth) {
+        uint32_t s = 0;
+        for (int k = 0; k < kWaves; ++k) s += h[k][threadIdx.x];
+        mb.dhist_part[blockIdx.x * (kMaxDepth + 1) + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_split_reduce(MixedBuffers mb)
+{
+    if (threadIdx.x > kMaxDepth) return;
+    uint32_t s = 0;
+    for (unsigned b = 0; b < kHistBlocks; ++b) s += mb.dhist_part[b * (kMaxDepth + 1) + threadIdx.x];
+    mb.dhist[threadIdx.x] = s;
 }
 
 __device__ inline double box_extent(const double* bx)
@@ -709,7 +735,8 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 
 void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s)
 {
-    k_split_hist<<<1, 1024, 0, s>>>(tb, mb);
+    k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb, mb);
+    k_split_reduce<<<1, 64, 0, s>>>(mb);
     k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb);
     scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
     k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);

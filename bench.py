@@ -147,11 +147,27 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain,mixed,learned",
+                    help="forms to run besides --form (comma list); the plain form gives speedup_vs_plain")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB ten times (calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE against known bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU (one GPU holds the whole problem); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch   # plumbing only: a plain copy kernel of known size
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 10}))
+        return
 
     bh = load_bh()
     forms = {"plain": bh.FORM_PLAIN, "mixed": bh.FORM_MIXED, "learned": bh.FORM_LEARNED}
@@ -160,8 +176,9 @@ def main():
     steps, warmup = args.steps, args.warmup
 
     results = {}
+    wanted = set(args.forms.split(",")) | {args.form}
     for name in ("plain", "mixed", "learned"):
-        if name != "plain" and name != args.form and not (args.form == "learned" and name == "mixed"):
+        if name not in wanted:
             continue
         log(f"bench: {name}, warmup {warmup}, steps {steps}")
         eng, wall, ph = run_form(bh, forms[name], pos, vel, mass, args.theta, steps, warmup)
@@ -189,7 +206,7 @@ def main():
 
     form = args.form
     head = results[form]
-    plain = results["plain"]
+    plain = results.get("plain", head)
     if form == "plain":
         kern, peak, bound_note = "k_walk_plain (FP64 walk + fused half kick)", FP64_PEAK_TFLOPS, "FP64 vector"
     else:
