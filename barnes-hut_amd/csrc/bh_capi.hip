@@ -146,6 +146,8 @@ int reserve(bh_ctx* c, size_t n)
     A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
     HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
+    HIP_TRY(c, hipMemsetAsync(t.scan_tmp, 0, bh::scan_tmp_words(2 * N) * sizeof(uint32_t), c->stream));
+    HIP_TRY(c, hipMemsetAsync(t.sort_tmp, 0, bh::sort_tmp_words(N) * sizeof(uint32_t), c->stream));
     A(t.sc, 1);
     A(c->partials, 6 * 1024);
 #undef A
@@ -173,7 +175,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb.sc, c->partials, s);
     bh::launch_keys(c->cur, c->n, c->tb.sc, c->tb.key, c->tb.perm, s);
     HIP_TRY(c, hipEventRecord(ev[1], s));
-    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63, c->tb.sort_tmp, s);
+    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63, c->tb.sort_tmp, c->cap, s);
     bh::launch_permute(c->cur, c->alt, c->tb.perm, c->tb.key, c->tb.flag, c->tb.heavy_slot, c->n, s);
     std::swap(c->cur, c->alt);
     HIP_TRY(c, hipEventRecord(ev[2], s));
@@ -240,6 +242,7 @@ int check_ctx(const bh_ctx* c) { return c ? BH_OK : BH_ERR_ARG; }
 
 constexpr uint64_t kSampleCapacity = 1u << 20;
 constexpr int64_t kModelIoRows = 1 << 16;   // rows per host model call
+static_assert(kModelIoRows <= bh::kMaxTrainBatch, "a gradient call evaluates all rows as one batch");
 
 template <class T>
 int malloc_model(bh_ctx* c, T** p, size_t count)
@@ -265,6 +268,8 @@ int model_alloc(bh_ctx* c)
     if ((rc = malloc_model(c, &d.n_seen, 1))) return rc;
     if ((rc = malloc_model(c, &d.last_loss, 1))) return rc;
     if ((rc = malloc_model(c, &d.io, (bh::kModelFeatures + 1) * kModelIoRows + bh::kModelParams + 1))) return rc;
+    if ((rc = malloc_model(c, &d.gpart, static_cast<size_t>(bh::kMaxTrainBatch / 64) * bh::kModelParams))) return rc;
+    if ((rc = malloc_model(c, &d.lpart, bh::kMaxTrainBatch / 64))) return rc;
     d.capacity = kSampleCapacity;
     return BH_OK;
 }
@@ -276,6 +281,10 @@ int check_device_errors(bh_ctx* c)
     HIP_TRY(c, hipMemcpy(&err, &c->tb.sc->error, sizeof err, hipMemcpyDeviceToHost));
     if (err & bh::kErrStack) return fail(c, BH_ERR_STATE, "mixed walk: traversal stack capacity exceeded");
     if (err) return fail(c, BH_ERR_STATE, "device guard raised error bits 0x%x", err);
+    uint32_t chain[2] = {0, 0};
+    HIP_TRY(c, hipMemcpy(&chain[0], bh::scan_error_word(c->tb.scan_tmp), 4, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(&chain[1], bh::sort_error_word(c->tb.sort_tmp), 4, hipMemcpyDeviceToHost));
+    if (chain[0] | chain[1]) return fail(c, BH_ERR_STATE, "scan/sort: a look-back wait gave up");
     return BH_OK;
 }
 
@@ -764,8 +773,8 @@ extern "C" {
 int bh_model_configure(bh_ctx* c, int train_steps, int iters_per_step, int batch, float lr)
 {
     if (check_ctx(c)) return BH_ERR_ARG;
-    if (train_steps < 0 || iters_per_step < 0 || batch <= 0 || !(lr > 0.f))
-        return fail(c, BH_ERR_ARG, "bad model configuration");
+    if (train_steps < 0 || iters_per_step < 0 || batch <= 0 || batch > bh::kMaxTrainBatch || !(lr > 0.f))
+        return fail(c, BH_ERR_ARG, "bad model configuration (batch 1..%d)", bh::kMaxTrainBatch);
     c->train_steps = train_steps;
     c->iters_per_step = iters_per_step;
     c->batch = batch;
@@ -860,7 +869,8 @@ int bh_model_gradients(bh_ctx* c, int64_t m, const float* features, const float*
 int bh_model_train(bh_ctx* c, int64_t m, const float* features, const float* targets, int iters, int batch,
                    float lr, float* final_loss)
 {
-    if (check_ctx(c) || m <= 0 || m > kModelIoRows || !features || !targets || iters < 0 || batch <= 0 || !(lr > 0))
+    if (check_ctx(c) || m <= 0 || m > kModelIoRows || !features || !targets || iters < 0 || batch <= 0 ||
+        batch > bh::kMaxTrainBatch || !(lr > 0))
         return fail(c, BH_ERR_ARG, "bh_model_train takes 1..%lld rows", (long long)kModelIoRows);
     float* dx = c->md.io;
     float* dt = dx + bh::kModelFeatures * kModelIoRows;

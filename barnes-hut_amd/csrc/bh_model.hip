@@ -11,7 +11,9 @@
 // mean squared error on y.
 // Training: Adam on the FP32 masters, BF16 weights refreshed after each
 // update; mini-batches drawn from a device ring buffer of (features, target)
-// samples filled after every walk of the training phase.
+// samples filled after every walk of the training phase.  One update is two
+// launches: per-block partial gradients over 64 samples each, then their sum
+// in block order and the Adam step.
 //
 // Parameter layout (kModelParams floats):
 //   W1[32][8] | b1[32] | W2[32][32] | b2[32] | W3[32] | b3
@@ -183,125 +185,151 @@ __device__ inline uint32_t mix32(uint32_t x)
     return x;
 }
 
-constexpr int kChunk = 64;    // samples per chunk held in LDS
+constexpr int kTile = 64;      // samples per block of the gradient kernel
+constexpr int kRow = kH + 1;   // padded LDS row: lanes of a wave read distinct banks
 
-// One block.  Gradients of the mean squared error over `batch` samples,
-// drawn (with a hash of iter) from the buffer or taken in order when
-// sample_mode == 0; then, if update, one Adam step on the FP32 masters and a
-// refresh of the BF16 weights.  grad_out / loss_out (optional) receive the
-// gradient and loss.
-__global__ __launch_bounds__(kNT) void k_model_train(ModelBuffers md, const float* __restrict__ xin,
-                                                     const float* __restrict__ tin, uint64_t m_in, int batch,
-                                                     int sample_mode, uint32_t iter, float lr, int update,
-                                                     float* __restrict__ grad_out, float* __restrict__ loss_out)
+// Gradient of the mean squared error over `batch` samples, 64 samples per
+// block.  Sample k of the batch is drawn with a hash of (iter, k) from the
+// buffer, or taken in order when sample_mode == 0.  Each block writes its
+// partial gradient (summed over its samples in sample order) to
+// gpart[block][kModelParams] and its partial loss to lpart[block];
+// k_model_apply adds the partials in block order, so the result does not
+// depend on timing.  Thread t works on sample t % 64 and on hidden units
+// 8 * (t / 64) .. +7 in the forward and backward passes.
+__global__ __launch_bounds__(kNT) void k_model_grad(ModelBuffers md, const float* __restrict__ xin,
+                                                    const float* __restrict__ tin, uint64_t m_in, int batch,
+                                                    int sample_mode, uint32_t iter, float* __restrict__ gpart,
+                                                    float* __restrict__ lpart)
 {
     __shared__ float P[kModelParams];
-    __shared__ float sx[kChunk][kF];
-    __shared__ float sh1[kChunk][kH];
-    __shared__ float sd2[kChunk][kH];
-    __shared__ float sd1[kChunk][kH];
-    __shared__ float sh2[kChunk][kH];
-    __shared__ float sdy[kChunk];
-    __shared__ float sloss[kNT];
+    __shared__ float sx[kTile][kF + 1];
+    __shared__ float z1[kTile][kRow], z2[kTile][kRow], d1[kTile][kRow], d2[kTile][kRow];
+    __shared__ float sdy[kTile], sl[kTile];
     load_params(md.wbf, P);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, s = t & (kTile - 1), jb = (t >> 6) * 8;
+    const int k = blockIdx.x * kTile + s;
+    const bool valid = k < batch;
     const float* X = xin ? xin : md.sfeat;
     const float* T = tin ? tin : md.starget;
     const uint64_t avail = xin ? m_in : min<uint64_t>(*md.n_seen, md.capacity);
-    float gacc[6] = {0, 0, 0, 0, 0, 0};   // params t, t+256, ... (<= 6 per thread)
-    float lsum = 0.f;
-    const float inv_b = 1.f / static_cast<float>(batch);
-    for (int c0 = 0; c0 < batch; c0 += kChunk) {
-        const int cn = min(kChunk, batch - c0);
-        if (t < cn && avail > 0) {
-            const int k = c0 + t;
-            const uint64_t s = sample_mode == 0 ? static_cast<uint64_t>(k) % avail
-                                                : static_cast<uint64_t>(mix32(iter * 0x9e3779b9u + k * 0x85ebca6bu + 1u)) % avail;
-            float x[kF];
+    const bool live = valid && avail > 0;
+    uint64_t row = 0;
+    if (live)
+        row = sample_mode == 0 ? static_cast<uint64_t>(k) % avail
+                               : static_cast<uint64_t>(mix32(iter * 0x9e3779b9u + static_cast<uint32_t>(k) * 0x85ebca6bu + 1u)) % avail;
+    if (t < kTile) {
 #pragma unroll
-            for (int f = 0; f < kF; ++f) {
-                x[f] = X[kF * s + f];
-                sx[t][f] = x[f];
-            }
-            Act a;
-            const float y = forward(P, x, a);
-            const float d = y - T[s];
-            lsum += d * d;
-            const float dy = 2.f * d * inv_b;
-            sdy[t] = dy;
-            float d2[kH];
-            for (int j = 0; j < kH; ++j) {
-                sh2[t][j] = relu(a.z2[j]);
-                sh1[t][j] = relu(a.z1[j]);
-                d2[j] = a.z2[j] > 0.f ? dy * P[kOffW3 + j] : 0.f;
-                sd2[t][j] = d2[j];
-            }
-            for (int k2 = 0; k2 < kH; ++k2) {
-                float s1 = 0.f;
-                for (int j = 0; j < kH; ++j) s1 = fmaf(d2[j], P[kOffW2 + j * kH + k2], s1);
-                sd1[t][k2] = a.z1[k2] > 0.f ? s1 : 0.f;
-            }
-        } else if (t < kChunk) {
-            sdy[t] = 0.f;
-            for (int j = 0; j < kH; ++j) sh2[t][j] = sh1[t][j] = sd2[t][j] = sd1[t][j] = 0.f;
-            for (int f = 0; f < kF; ++f) sx[t][f] = 0.f;
-        }
-        __syncthreads();
-        for (int q = 0; q < 6; ++q) {
-            const int pidx = t + q * kNT;
-            if (pidx >= kModelParams) break;
-            float acc = 0.f;
-            if (pidx < kOffB1) {
-                const int j = pidx / kF, f = pidx % kF;
-                for (int s = 0; s < cn; ++s) acc = fmaf(sd1[s][j], sx[s][f], acc);
-            } else if (pidx < kOffW2) {
-                const int j = pidx - kOffB1;
-                for (int s = 0; s < cn; ++s) acc += sd1[s][j];
-            } else if (pidx < kOffB2) {
-                const int j = (pidx - kOffW2) / kH, k2 = (pidx - kOffW2) % kH;
-                for (int s = 0; s < cn; ++s) acc = fmaf(sd2[s][j], sh1[s][k2], acc);
-            } else if (pidx < kOffW3) {
-                const int j = pidx - kOffB2;
-                for (int s = 0; s < cn; ++s) acc += sd2[s][j];
-            } else if (pidx < kOffB3) {
-                const int j = pidx - kOffW3;
-                for (int s = 0; s < cn; ++s) acc = fmaf(sdy[s], sh2[s][j], acc);
-            } else {
-                for (int s = 0; s < cn; ++s) acc += sdy[s];
-            }
-            gacc[q] += acc;
-        }
-        __syncthreads();
+        for (int f = 0; f < kF; ++f) sx[s][f] = live ? X[kF * row + f] : 0.f;
     }
-    sloss[t] = lsum;
     __syncthreads();
+    // layer 1
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int j = jb + q;
+        float a = P[kOffB1 + j];
+#pragma unroll
+        for (int f = 0; f < kF; ++f) a = fmaf(P[kOffW1 + j * kF + f], sx[s][f], a);
+        z1[s][j] = a;
+    }
+    __syncthreads();
+    // layer 2
+    for (int q = 0; q < 8; ++q) {
+        const int j = jb + q;
+        float a = P[kOffB2 + j];
+#pragma unroll 8
+        for (int i = 0; i < kH; ++i) a = fmaf(P[kOffW2 + j * kH + i], relu(z1[s][i]), a);
+        z2[s][j] = a;
+    }
+    __syncthreads();
+    // output, residual on feature 5, loss and dL/dy
+    if (t < kTile) {
+        float y = P[kOffB3];
+#pragma unroll 8
+        for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(z2[s][j]), y);
+        y += sx[s][5];
+        const float d = live ? y - T[row] : 0.f;
+        sl[s] = d * d;
+        sdy[s] = 2.f * d / static_cast<float>(batch);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int j = jb + q;
+        d2[s][j] = z2[s][j] > 0.f ? sdy[s] * P[kOffW3 + j] : 0.f;
+    }
+    __syncthreads();
+    for (int q = 0; q < 8; ++q) {
+        const int i = jb + q;
+        float a = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < kH; ++j) a = fmaf(d2[s][j], P[kOffW2 + j * kH + i], a);
+        d1[s][i] = z1[s][i] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    // parameter gradients of this block's samples, in sample order
+    float* g = gpart + static_cast<size_t>(blockIdx.x) * kModelParams;
+    for (int p = t; p < kModelParams; p += kNT) {
+        float a = 0.f;
+        if (p < kOffB1) {
+            const int j = p / kF, f = p % kF;
+            for (int r = 0; r < kTile; ++r) a = fmaf(d1[r][j], sx[r][f], a);
+        } else if (p < kOffW2) {
+            const int j = p - kOffB1;
+            for (int r = 0; r < kTile; ++r) a += d1[r][j];
+        } else if (p < kOffB2) {
+            const int j = (p - kOffW2) / kH, i = (p - kOffW2) % kH;
+            for (int r = 0; r < kTile; ++r) a = fmaf(d2[r][j], relu(z1[r][i]), a);
+        } else if (p < kOffW3) {
+            const int j = p - kOffB2;
+            for (int r = 0; r < kTile; ++r) a += d2[r][j];
+        } else if (p < kOffB3) {
+            const int j = p - kOffW3;
+            for (int r = 0; r < kTile; ++r) a = fmaf(sdy[r], relu(z2[r][j]), a);
+        } else {
+            for (int r = 0; r < kTile; ++r) a += sdy[r];
+        }
+        g[p] = a;
+    }
     if (t == 0) {
         float l = 0.f;
-        for (int k = 0; k < kNT; ++k) l += sloss[k];
-        l *= inv_b;
+        for (int r = 0; r < kTile; ++r) l += sl[r];
+        lpart[blockIdx.x] = l;
+    }
+}
+
+// Sums the nb partial gradients (block order) and, if update, takes one Adam
+// step (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) on the FP32
+// masters and refreshes the BF16 weights.  grad_out / loss_out (optional)
+// receive the gradient and the mean loss.
+__global__ __launch_bounds__(kNT) void k_model_apply(ModelBuffers md, const float* __restrict__ gpart,
+                                                     const float* __restrict__ lpart, int nb, int batch, uint32_t iter,
+                                                     float lr, int update, float* __restrict__ grad_out,
+                                                     float* __restrict__ loss_out)
+{
+    const int p = blockIdx.x * kNT + threadIdx.x;
+    if (p == 0) {
+        float l = 0.f;
+        for (int b = 0; b < nb; ++b) l += lpart[b];
+        l /= static_cast<float>(batch);
         if (loss_out) *loss_out = l;
         *md.last_loss = l;
     }
-    for (int q = 0; q < 6; ++q) {
-        const int pidx = t + q * kNT;
-        if (pidx >= kModelParams) break;
-        const float g = gacc[q];
-        if (grad_out) grad_out[pidx] = g;
-        if (update) {
-            // Adam (beta1 0.9, beta2 0.999, eps 1e-8) with bias correction
-            const float b1 = 0.9f, b2 = 0.999f;
-            const float mm = b1 * md.adam_m[pidx] + (1.f - b1) * g;
-            const float vv = b2 * md.adam_v[pidx] + (1.f - b2) * g * g;
-            md.adam_m[pidx] = mm;
-            md.adam_v[pidx] = vv;
-            const float step = static_cast<float>(iter + 1);
-            const float mh = mm / (1.f - powf(b1, step));
-            const float vh = vv / (1.f - powf(b2, step));
-            const float w = md.wf32[pidx] - lr * mh / (sqrtf(vh) + 1e-8f);
-            md.wf32[pidx] = w;
-            md.wbf[pidx] = to_bf16(w);
-        }
-    }
+    if (p >= kModelParams) return;
+    float g = 0.f;
+    for (int b = 0; b < nb; ++b) g += gpart[static_cast<size_t>(b) * kModelParams + p];
+    if (grad_out) grad_out[p] = g;
+    if (!update) return;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float mm = b1 * md.adam_m[p] + (1.f - b1) * g;
+    const float vv = b2 * md.adam_v[p] + (1.f - b2) * g * g;
+    md.adam_m[p] = mm;
+    md.adam_v[p] = vv;
+    const float step = static_cast<float>(iter + 1);
+    const float mh = mm / (1.f - powf(b1, step));
+    const float vh = vv / (1.f - powf(b2, step));
+    const float w = md.wf32[p] - lr * mh / (sqrtf(vh) + 1e-8f);
+    md.wf32[p] = w;
+    md.wbf[p] = to_bf16(w);
 }
 
 __global__ void k_refresh_bf16(ModelBuffers md)
@@ -342,7 +370,10 @@ void launch_model_train(const ModelBuffers& md, const float* x, const float* t, 
                         int sample_mode, uint32_t iter, float lr, bool update, float* grad_out, float* loss_out,
                         hipStream_t s)
 {
-    k_model_train<<<1, kNT, 0, s>>>(md, x, t, m, batch, sample_mode, iter, lr, update ? 1 : 0, grad_out, loss_out);
+    const int nb = (batch + kTile - 1) / kTile;   // batch <= kMaxTrainBatch, checked by the callers
+    k_model_grad<<<nb, kNT, 0, s>>>(md, x, t, m, batch, sample_mode, iter, md.gpart, md.lpart);
+    k_model_apply<<<(kModelParams + kNT - 1) / kNT, kNT, 0, s>>>(md, md.gpart, md.lpart, nb, batch, iter, lr,
+                                                                 update ? 1 : 0, grad_out, loss_out);
 }
 
 void launch_refresh_bf16(const ModelBuffers& md, hipStream_t s) { k_refresh_bf16<<<1, kNT, 0, s>>>(md); }

@@ -1,15 +1,29 @@
 // bh_sort.hip -- device-wide exclusive scan and a stable LSD radix sort of
-// 64-bit Morton keys with a 32-bit payload, both staged through LDS.
+// 64-bit Morton keys with a 32-bit payload, both single-pass over their data.
 //
-// Scan: tiles of 2048 values (256 threads x 8), three kernels: tile sums,
-// one-block scan of the tile sums, tile-local scan plus the tile offset.
+// Tiles are chained by decoupled look-back: a block takes the next tile id
+// from an atomic counter, publishes its tile's aggregate, walks back over the
+// status words of the tiles before it until it finds an inclusive prefix, and
+// publishes its own inclusive prefix.  A status word is 64 bits -- tag
+// ((epoch + 1) << 1 | inclusive) over the 32-bit value -- written and read
+// with relaxed device-scope atomics, so the value travels with its flag and
+// no cache flush is needed.  The epoch advances once per launch (the last
+// block to finish bumps it and resets the counters), so words left by earlier
+// launches never match.  A tile only waits for tiles with smaller ids, which
+// were handed to blocks that are already running and publish their
+// aggregates before they wait for anything; each wait is still bounded and
+// sets the chain's error word (read by the host after the step) instead of
+// spinning forever.
 //
-// Sort: 8-bit digits, one pass per digit.  Each pass is a per-tile digit
-// histogram (LDS atomics), a scan of the digit-major histogram table, and a
-// scatter in which every wavefront ranks its 512 keys with ballots (the peers
-// of a lane are the lanes that share its digit) and a running per-wave digit
-// count in LDS, so that equal digits keep their input order: the sort is
-// stable, which fixes the order of equal Morton keys.
+// Scan: tiles of 2048 values (256 threads x 8), one launch.
+//
+// Sort: 8-bit digits.  One launch counts the digits of every pass (per-block
+// LDS histograms, reduced and scanned by the last block), then one launch
+// per pass: every wavefront ranks its 512 keys with ballots (the peers of a
+// lane are the lanes that share its digit) and a running per-wave digit count
+// in LDS, so that equal digits keep their input order -- the sort is stable,
+// which fixes the order of equal Morton keys -- and the tile's 256 digit
+// counts are chained across tiles by look-back.
 #include <hip/hip_runtime.h>
 
 #include "bh_internal.hpp"
@@ -21,6 +35,103 @@ constexpr int kNT = 256;
 constexpr int kIPT = 8;
 constexpr int kTile = kNT * kIPT;  // 2048
 constexpr int kWaves = kNT / 64;
+constexpr int kDigits = 256;
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// look-back state at the head of a tmp buffer: counters, then status words
+struct Chain {
+    uint32_t* ctr;       // [0] next tile id, [1] finished blocks, [2] epoch, [3] error
+    uint64_t* status;    // width words per tile
+};
+constexpr size_t kChainHeadWords = 4;
+
+__device__ inline Chain chain_at(uint32_t* tmp)
+{
+    return Chain{tmp, reinterpret_cast<uint64_t*>(tmp + kChainHeadWords)};
+}
+
+__device__ inline uint64_t status_word(uint32_t epoch, bool inclusive, uint32_t v)
+{
+    const uint32_t tag = (((epoch + 1u) & 0x7fffffffu) << 1) | (inclusive ? 1u : 0u);
+    return (static_cast<uint64_t>(tag) << 32) | v;
+}
+
+__device__ inline void publish(uint64_t* w, uint64_t v)
+{
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sum of the values of tiles [0, tile) in column d of a chain of the given
+// width.  The status words of up to 8 predecessors are loaded at once, so a
+// walk over aggregates costs one load latency per 8 tiles.
+constexpr int kLookWindow = 8;
+
+__device__ inline uint32_t look_back(const Chain& c, uint32_t tile, uint32_t width, uint32_t d, uint32_t epoch)
+{
+    const uint32_t want = ((epoch + 1u) & 0x7fffffffu);
+    uint32_t excl = 0, spins = 0;
+    int64_t k = static_cast<int64_t>(tile) - 1;
+    while (k >= 0) {
+        uint64_t s[kLookWindow];
+#pragma unroll
+        for (int i = 0; i < kLookWindow; ++i)
+            s[i] = (k - i >= 0) ? __hip_atomic_load(c.status + static_cast<size_t>(k - i) * width + d,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int i = 0; i < kLookWindow; ++i) {
+            if (done || used != i || k - i < 0) continue;
+            const uint32_t tag = static_cast<uint32_t>(s[i] >> 32);
+            if ((tag >> 1) != want) continue;   // not published yet: reload from here
+            excl += static_cast<uint32_t>(s[i]);
+            used = i + 1;
+            if (tag & 1u) done = true;
+        }
+        if (done) break;
+        if (used > 0) {
+            k -= used;
+            spins = 0;
+        } else if (++spins > kSpinLimit) {
+            atomicOr(c.ctr + 3, 1u);
+            break;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return excl;
+}
+
+// Thread 0 takes the next tile id; every thread gets (id, epoch).  An id
+// beyond the grid (counters not reset, which a completed launch always does)
+// raises the error word; the caller then skips its tile.
+__device__ inline bool chain_enter(const Chain& c, uint32_t* s_id, uint32_t* s_epoch)
+{
+    if (threadIdx.x == 0) {
+        *s_id = atomicAdd(c.ctr, 1u);
+        *s_epoch = __hip_atomic_load(c.ctr + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (*s_id >= gridDim.x) atomicOr(c.ctr + 3, 2u);
+    }
+    __syncthreads();
+    return *s_id < gridDim.x;
+}
+
+// After the block's last status write: the last block to finish resets the
+// counters and advances the epoch for the next launch.
+__device__ inline void chain_leave(const Chain& c, uint32_t epoch)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev = atomicAdd(c.ctr + 1, 1u);
+        if (prev == gridDim.x - 1) {
+            __hip_atomic_store(c.ctr + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(c.ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(c.ctr + 2, epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v)
 {
@@ -56,52 +167,20 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uin
     return r;
 }
 
-__global__ __launch_bounds__(kNT) void k_scan_tile_sums(const uint32_t* __restrict__ in, size_t n,
-                                                        uint32_t* __restrict__ sums)
+// Single-pass exclusive scan; in and out may alias (a tile reads its values
+// before it writes them).
+__global__ __launch_bounds__(kNT) void k_scan(const uint32_t* in, uint32_t* out, size_t n,
+                                              uint32_t* __restrict__ total_out, uint32_t* tmp)
 {
     __shared__ uint32_t lds[kWaves + 1];
-    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kIPT; ++j)
-        if (base + j < n) s += in[base + j];
-    uint32_t total;
-    block_exclusive_scan(s, &total, lds);
-    if (threadIdx.x == 0) sums[blockIdx.x] = total;
-}
-
-__global__ __launch_bounds__(kNT) void k_scan_sums(uint32_t* __restrict__ sums, size_t nt,
-                                                   uint32_t* __restrict__ total_out)
-{
-    __shared__ uint32_t lds[kWaves + 1];
-    uint32_t carry = 0;
-    for (size_t base = 0; base < nt; base += kTile) {
-        const size_t b = base + static_cast<size_t>(threadIdx.x) * kIPT;
-        uint32_t v[kIPT];
-        uint32_t s = 0;
-#pragma unroll
-        for (int j = 0; j < kIPT; ++j) {
-            v[j] = (b + j < nt) ? sums[b + j] : 0u;
-            s += v[j];
-        }
-        uint32_t total;
-        uint32_t ex = block_exclusive_scan(s, &total, lds) + carry;
-#pragma unroll
-        for (int j = 0; j < kIPT; ++j) {
-            if (b + j < nt) sums[b + j] = ex;
-            ex += v[j];
-        }
-        carry += total;
+    __shared__ uint32_t s_id, s_epoch, s_excl;
+    const Chain c = chain_at(tmp);
+    if (!chain_enter(c, &s_id, &s_epoch)) {
+        chain_leave(c, s_epoch);
+        return;
     }
-    if (threadIdx.x == 0 && total_out) *total_out = carry;
-}
-
-__global__ __launch_bounds__(kNT) void k_scan_apply(const uint32_t* __restrict__ in, size_t n,
-                                                    const uint32_t* __restrict__ sums,
-                                                    uint32_t* __restrict__ out)
-{
-    __shared__ uint32_t lds[kWaves + 1];
-    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
+    const uint32_t tile = s_id, epoch = s_epoch;
+    const size_t base = static_cast<size_t>(tile) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
     uint32_t v[kIPT];
     uint32_t s = 0;
 #pragma unroll
@@ -109,48 +188,125 @@ __global__ __launch_bounds__(kNT) void k_scan_apply(const uint32_t* __restrict__
         v[j] = (base + j < n) ? in[base + j] : 0u;
         s += v[j];
     }
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(s, &total, lds) + sums[blockIdx.x];
+    uint32_t agg;
+    uint32_t ex = block_exclusive_scan(s, &agg, lds);
+    if (threadIdx.x == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            publish(c.status, status_word(epoch, true, agg));
+        } else {
+            publish(c.status + tile, status_word(epoch, false, agg));
+            excl = look_back(c, tile, 1, 0, epoch);
+            publish(c.status + tile, status_word(epoch, true, excl + agg));
+        }
+        if (tile == gridDim.x - 1 && total_out) *total_out = excl + agg;
+        s_excl = excl;
+    }
+    __syncthreads();
+    ex += s_excl;
 #pragma unroll
     for (int j = 0; j < kIPT; ++j) {
         if (base + j < n) out[base + j] = ex;
         ex += v[j];
     }
+    chain_leave(c, epoch);
 }
 
 // ------------------------------------------------------------ radix sort
+// sort tmp layout (u32 words), every offset fixed by the capacity so that
+// calls with different n find the same counters: finished-block counter of
+// the histogram launch (4 words) | digit base [8][256] | digit counts
+// [8][256] | look-back chain head (4 words) and status words (2 u32 each,
+// 256 per tile, sort_tiles(cap) tiles)
+struct SortTmp {
+    uint32_t* done;
+    uint32_t* base;
+    uint32_t* count;
+    uint32_t* chain;
+};
 
-__global__ __launch_bounds__(kNT) void k_sort_hist(const uint64_t* __restrict__ key, size_t n, int shift,
-                                                   uint32_t* __restrict__ counts, uint32_t nblocks)
+__host__ __device__ inline size_t sort_tiles(size_t n) { return (n + kTile - 1) / kTile > 0 ? (n + kTile - 1) / kTile : 1; }
+
+__host__ __device__ inline SortTmp sort_tmp_at(uint32_t* tmp)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    SortTmp t;
+    t.done = tmp;
+    t.base = t.done + 4;
+    t.count = t.base + kMaxPasses * kDigits;
+    t.chain = t.count + kMaxPasses * kDigits;
+    return t;
+}
+
+// Digit counts of every pass: per-block LDS histograms added into the global
+// counts; the last block turns them into each pass's exclusive digit offsets
+// (base) and clears the counts and its counter for the next sort.
+__global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restrict__ key, size_t n, int passes,
+                                                       uint32_t* tmp)
+{
+    __shared__ uint32_t h[kMaxPasses][kDigits];
+    __shared__ bool last;
+    const SortTmp st = sort_tmp_at(tmp);
+    for (int k = threadIdx.x; k < kMaxPasses * kDigits; k += kNT) (&h[0][0])[k] = 0;
     __syncthreads();
     const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
 #pragma unroll
     for (int j = 0; j < kIPT; ++j) {
-        size_t idx = base + static_cast<size_t>(j) * kNT + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(key[idx] >> shift) & 255u], 1u);
+        const size_t idx = base + static_cast<size_t>(j) * kNT + threadIdx.x;
+        if (idx < n) {
+            const uint64_t k = key[idx];
+            for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+        }
     }
     __syncthreads();
-    counts[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int k = threadIdx.x; k < passes * kDigits; k += kNT) {
+        const uint32_t v = (&h[0][0])[k];
+        if (v) atomicAdd(st.count + k, v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t prev = atomicAdd(st.done, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int k = threadIdx.x; k < passes * kDigits; k += kNT) {
+        (&h[0][0])[k] = __hip_atomic_load(st.count + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(st.count + k, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (threadIdx.x < passes) {
+        uint32_t run = 0;
+        for (int d = 0; d < kDigits; ++d) {
+            const uint32_t t = h[threadIdx.x][d];
+            st.base[threadIdx.x * kDigits + d] = run;
+            run += t;
+        }
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(st.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kNT) void k_sort_scatter(const uint64_t* __restrict__ kin,
-                                                      const uint32_t* __restrict__ vin,
-                                                      uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                      size_t n, int shift, const uint32_t* __restrict__ offsets,
-                                                      uint32_t nblocks)
+__global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                   uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
+                                                   int pass, uint32_t* tmp)
 {
-    __shared__ uint32_t wcnt[kWaves][256];
-    __shared__ uint32_t boff[256];
+    __shared__ uint32_t wcnt[kWaves][kDigits];
+    __shared__ uint32_t boff[kDigits];
+    __shared__ uint32_t s_id, s_epoch;
+    const SortTmp st = sort_tmp_at(tmp);
+    const Chain c = chain_at(st.chain);
+    const int shift = 8 * pass;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int k = 0; k < kWaves; ++k) wcnt[k][threadIdx.x] = 0;
-    boff[threadIdx.x] = offsets[static_cast<size_t>(threadIdx.x) * nblocks + blockIdx.x];
-    __syncthreads();
+    if (!chain_enter(c, &s_id, &s_epoch)) {   // includes the barrier after the clears
+        chain_leave(c, s_epoch);
+        return;
+    }
+    const uint32_t tile = s_id, epoch = s_epoch;
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const size_t wbase = static_cast<size_t>(blockIdx.x) * kTile + static_cast<size_t>(w) * (kIPT * 64);
+    const size_t wbase = static_cast<size_t>(tile) * kTile + static_cast<size_t>(w) * (kIPT * 64);
     uint64_t kk[kIPT];
     uint32_t vv[kIPT];
     uint32_t local[kIPT];
@@ -180,13 +336,25 @@ __global__ __launch_bounds__(kNT) void k_sort_scatter(const uint64_t* __restrict
     }
     __syncthreads();
     {
-        // exclusive scan of each digit's count over the waves of the block
+        // exclusive scan of each digit's count over the waves of the block;
+        // thread d then chains the tile's count of digit d across tiles
+        const int d = threadIdx.x;
         uint32_t run = 0;
         for (int k = 0; k < kWaves; ++k) {
-            uint32_t t = wcnt[k][threadIdx.x];
-            wcnt[k][threadIdx.x] = run;
+            const uint32_t t = wcnt[k][d];
+            wcnt[k][d] = run;
             run += t;
         }
+        uint64_t* mine = c.status + static_cast<size_t>(tile) * kDigits + d;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            publish(mine, status_word(epoch, true, run));
+        } else {
+            publish(mine, status_word(epoch, false, run));
+            excl = look_back(c, tile, kDigits, d, epoch);
+            publish(mine, status_word(epoch, true, excl + run));
+        }
+        boff[d] = st.base[pass * kDigits + d] + excl;
     }
     __syncthreads();
 #pragma unroll
@@ -199,52 +367,43 @@ __global__ __launch_bounds__(kNT) void k_sort_scatter(const uint64_t* __restrict
             vout[pos] = vv[j];
         }
     }
+    chain_leave(c, epoch);
 }
 
 }  // namespace
 
-size_t scan_tmp_words(size_t n)
-{
-    size_t nt = (n + kTile - 1) / kTile;
-    return nt + 1;
-}
+size_t scan_tmp_words(size_t cap) { return kChainHeadWords + 2 * sort_tiles(cap); }
 
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
                         hipStream_t s)
 {
     size_t nt = (n + kTile - 1) / kTile;
     if (nt == 0) nt = 1;
-    k_scan_tile_sums<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp);
-    k_scan_sums<<<1, kNT, 0, s>>>(tmp, nt, total);
-    k_scan_apply<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp, out);
+    k_scan<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, out, n, total, tmp);
 }
 
-size_t sort_tmp_words(size_t n)
+size_t sort_tmp_words(size_t cap)
 {
-    size_t nb = (n + kTile - 1) / kTile;
-    if (nb == 0) nb = 1;
-    size_t table = 256 * nb;
-    return table + scan_tmp_words(table) + 1;
+    return 4 + 2 * kMaxPasses * kDigits + kChainHeadWords + 2 * kDigits * sort_tiles(cap);
 }
+
+uint32_t* scan_error_word(uint32_t* scan_tmp) { return scan_tmp + 3; }
+
+uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
 
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* tmp, hipStream_t s)
+                      uint32_t* tmp, size_t cap, hipStream_t s)
 {
-    if (n == 0) return;
-    const uint32_t nb = static_cast<uint32_t>((n + kTile - 1) / kTile);
-    const size_t table = 256 * static_cast<size_t>(nb);
-    uint32_t* counts = tmp;
-    uint32_t* stmp = tmp + table;
-    const int passes = (bits + 7) / 8;
+    if (n == 0 || n > cap) return;
+    const int passes = (bits + 7) / 8;   // <= 8
+    const unsigned nt = static_cast<unsigned>(sort_tiles(n));
+    k_sort_hist_all<<<nt, kNT, 0, s>>>(k0, n, passes, tmp);
     uint64_t* ka = k0;
     uint32_t* va = v0;
     uint64_t* kb = k1;
     uint32_t* vb = v1;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        k_sort_hist<<<nb, kNT, 0, s>>>(ka, n, shift, counts, nb);
-        scan_exclusive_u32(counts, counts, nullptr, table, stmp, s);
-        k_sort_scatter<<<nb, kNT, 0, s>>>(ka, va, kb, vb, n, shift, counts, nb);
+        k_sort_pass<<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, tmp);
         uint64_t* tk = ka;
         ka = kb;
         kb = tk;

@@ -388,12 +388,11 @@ __global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
     tb.child[8 * static_cast<size_t>(pb) + oct] = static_cast<int32_t>(t);
 }
 
-// raw moments of the branching nodes of one depth (src/linear_octree.cpp:69-87)
-__global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb, int depth)
+// raw moments of branching node bi from its existing children, summed in
+// octant order 0..7 from 0.0 (src/linear_octree.cpp:69-87); the light
+// moments and light-body boxes alongside when the mixed forms need them
+__device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi)
 {
-    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
-    if (bi >= B || tb.bdepth[bi] != depth) return;
     double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     const int32_t* c = tb.child + 8 * bi;
 #pragma unroll
@@ -440,6 +439,16 @@ __global__ __launch_bounds__(kNT) void k_moments_level(size_t n, TreeBuffers tb,
         q[2] = ly;
         q[3] = lz;
     }
+}
+
+// raw moments of the branching nodes of one depth; levels run bottom-up, one
+// launch each, so every child is final before its parent reads it
+__global__ __launch_bounds__(kNT) void k_moments_level(TreeBuffers tb, int depth)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (bi >= B || tb.bdepth[bi] != depth) return;
+    node_moments(tb, M, bi);
 }
 
 // reversed per-leaf node counts for the suffix scan
@@ -555,7 +564,7 @@ void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s
     scan_exclusive_u32(tb.is_rep, tb.bidx, &tb.sc->n_branch, n, tb.scan_tmp, s);
     k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
     k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
-    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(n, tb, d);
+    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
 }
 
 void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s)
