@@ -48,6 +48,11 @@ class Config(C.Structure):
                 ("theta", C.c_double), ("G", C.c_double), ("soft2", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("gmax", C.c_int), ("efac", C.c_double), ("occ", C.c_int), ("walk_blocks", C.c_int),
+                ("best_ms", C.c_double), ("default_ms", C.c_double), ("n_tried", C.c_int)]
+
+
 _P = C.c_void_p
 _I64 = C.c_int64
 _D = C.c_double
@@ -91,6 +96,13 @@ SIGNATURES = [
     ("bh_generate_ensemble_sizes", C.c_int, [C.c_uint64, _I64, _I64, _I64, _P]),
     ("bh_write_csv", C.c_int, [C.c_char_p, _I64, _P, _P, _P]),
     ("bh_read_csv", C.c_int, [C.c_char_p, _I64, _I64, C.POINTER(_I64), _P, _P, _P, _P]),
+    ("bh_set_graph", C.c_int, [_P, C.c_int]),
+    ("bh_get_graph_replays", C.c_int, [_P, C.POINTER(_I64)]),
+    ("bh_set_systems", C.c_int, [_P, _I64, _P]),
+    ("bh_get_systems", C.c_int, [_P, C.POINTER(_I64), _I64, _P]),
+    ("bh_set_group_params", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int]),
+    ("bh_get_group_params", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_D), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("bh_autotune", C.c_int, [_P, C.c_int, C.c_int, _P]),
 ]
 
 _lib = None
@@ -198,6 +210,7 @@ class Engine:
             raise BHError(rc, self.lib.bh_last_error(None).decode())
         self.h = h
         self.n = 0
+        self.form = form
 
     def close(self):
         if getattr(self, "h", None):
@@ -295,6 +308,42 @@ class Engine:
 
     def set_schedule(self, mode: int):
         self._ck(self.lib.bh_set_schedule(self.h, mode))
+
+    # ------------------------------------------------ graphs, ensembles, tuning
+    def set_graph(self, on: bool):
+        self._ck(self.lib.bh_set_graph(self.h, 1 if on else 0))
+
+    def graph_replays(self) -> int:
+        r = _I64(0)
+        self._ck(self.lib.bh_get_graph_replays(self.h, C.byref(r)))
+        return r.value
+
+    def set_systems(self, counts):
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        self._ck(self.lib.bh_set_systems(self.h, len(counts), _ptr(counts)))
+
+    def systems(self) -> np.ndarray:
+        """Body offsets of the systems (S + 1 entries)."""
+        k = _I64(0)
+        self._ck(self.lib.bh_get_systems(self.h, C.byref(k), 0, None))
+        out = np.zeros(k.value + 1, np.int64)
+        self._ck(self.lib.bh_get_systems(self.h, C.byref(k), len(out), _ptr(out)))
+        return out
+
+    def set_group_params(self, gmax=64, efac=2.0, occ=32, walk_blocks=0):
+        self._ck(self.lib.bh_set_group_params(self.h, gmax, efac, occ, walk_blocks))
+
+    def group_params(self) -> dict:
+        g, o, w = C.c_int(0), C.c_int(0), C.c_int(0)
+        e = _D(0)
+        self._ck(self.lib.bh_get_group_params(self.h, C.byref(g), C.byref(e), C.byref(o), C.byref(w)))
+        return {"gmax": g.value, "efac": e.value, "occ": o.value, "walk_blocks": w.value}
+
+    def autotune(self, form: int = None, reps: int = 3) -> dict:
+        r = TuneResult()
+        self._ck(self.lib.bh_autotune(self.h, self.form if form is None else form, reps, C.byref(r)))
+        return {"gmax": r.gmax, "efac": r.efac, "occ": r.occ, "walk_blocks": r.walk_blocks,
+                "best_ms": r.best_ms, "default_ms": r.default_ms, "n_tried": r.n_tried}
 
     # ------------------------------------------------ learned cost model
     MODEL_PARAMS = 1377

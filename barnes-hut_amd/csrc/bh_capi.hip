@@ -5,7 +5,11 @@
 // stream with no host round trip:
 //   kick/drift/box -> keys -> radix sort -> gather -> tree -> walk (+ kick)
 // so a step costs one launch sequence and the host only waits when the
-// caller asks for data.
+// caller asks for data.  bh_step captures the steady-state step (both
+// parities of the body double buffer) into HIP graphs and replays them.
+//
+// Systems: the bodies form n_sys independent systems, each a fixed range of
+// the body arrays (bh_set_systems); one system is the default.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,8 +40,20 @@ struct bh_ctx {
     Bodies cur{}, alt{};
     TreeBuffers tb{};
     MixedBuffers mb{};
-    int n_heavy = 0;
-    unsigned mixed_blocks = 0;
+    uint32_t n_sys = 1;
+    std::vector<int64_t> sys_off;     // n_sys + 1 body offsets
+    uint8_t* bsys_w = nullptr;        // device: system of each body position
+    uint32_t heavy_items = 0;         // work items of the heavy direct sums
+    size_t heavy_item_cap = 0;
+    bh::GroupParams gp{};
+    unsigned mixed_blocks = 0;        // persistent walk blocks in use
+    unsigned mixed_blocks_max = 0;    // occupancy limit of the device
+    // graph replay of bh_step: one graph per parity of the body double buffer
+    bool use_graph = true;
+    int parity = 0;                   // toggles with every cur/alt swap
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<double> gkey[2];      // launch parameters each graph was captured with
+    int64_t graph_replays = 0;
     int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
     bool model_ready = false;         // the LEARNED cost model is trained
     // cost model (allocated once per context, independent of N)
@@ -87,8 +103,18 @@ int fail(bh_ctx* c, int code, const char* fmt, ...)
             return fail((ctx), BH_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
     } while (0)
 
+void drop_graphs(bh_ctx* c)
+{
+    for (int k = 0; k < 2; ++k) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        c->gexec[k] = nullptr;
+        c->gkey[k].clear();
+    }
+}
+
 void free_buffers(bh_ctx* c)
 {
+    drop_graphs(c);
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->cur = Bodies{};
@@ -98,6 +124,7 @@ void free_buffers(bh_ctx* c)
     c->mb = keep;
     c->md.feat = nullptr;
     c->partials = nullptr;
+    c->bsys_w = nullptr;
     c->cnt_a = c->cnt_o = c->cnt_t = nullptr;
     c->cap = 0;
     c->have_tree = false;
@@ -138,18 +165,21 @@ int reserve(bh_ctx* c, size_t n)
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
     A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.walk, 2 * N);
-    A(t.heavy_slot, 64); A(t.gflag, N);
+    A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems);
+    t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
     A(m.g_inter, N); A(m.g_visit, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
-    A(m.sched_hist, 64); A(m.sched_cursor, 64); A(m.hpart, 3 * bh::kMaxHeavy * bh::kHeavyChunks); A(m.dhist, 32); A(m.dhist_part, 128 * 32);
+    A(m.sched_hist, 64); A(m.sched_cursor, 64);
+    c->heavy_item_cap = static_cast<size_t>(bh::kMaxHeavy) * (N / bh::kHeavyChunk + bh::kMaxSystems + 1);
+    A(m.hpart, 3 * c->heavy_item_cap);
     A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
     HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
     HIP_TRY(c, hipMemsetAsync(t.scan_tmp, 0, bh::scan_tmp_words(2 * N) * sizeof(uint32_t), c->stream));
     HIP_TRY(c, hipMemsetAsync(t.sort_tmp, 0, bh::sort_tmp_words(N) * sizeof(uint32_t), c->stream));
     A(t.sc, 1);
-    A(c->partials, 6 * 1024);
+    A(c->partials, 6 * bh::kBoxBlocks);
 #undef A
     HIP_TRY(c, hipMemsetAsync(t.sc, 0, sizeof(StepScalars), c->stream));
     c->cap = N;
@@ -169,16 +199,23 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     const bool mixed = form != BH_FORM_PLAIN;
     if (mixed && c->n >= (size_t{1} << 26))
         return fail(c, BH_ERR_ARG, "the mixed forms take at most 2^26 - 1 bodies");
+    if (!mixed && c->n_sys > 1)
+        return fail(c, BH_ERR_UNSUPPORTED, "the plain form runs one system; ensembles use the mixed forms");
     hipStream_t s = c->stream;
     const double half_dt = 0.5 * dt;
-    HIP_TRY(c, hipEventRecord(ev[0], s));
-    bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb.sc, c->partials, s);
-    bh::launch_keys(c->cur, c->n, c->tb.sc, c->tb.key, c->tb.perm, s);
-    HIP_TRY(c, hipEventRecord(ev[1], s));
-    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63, c->tb.sort_tmp, c->cap, s);
-    bh::launch_permute(c->cur, c->alt, c->tb.perm, c->tb.key, c->tb.flag, c->tb.heavy_slot, c->n, s);
+    // ev == nullptr while a step is captured into a graph: no phase events then
+#define MARK(k) \
+    if (ev) HIP_TRY(c, hipEventRecord(ev[k], s))
+    MARK(0);
+    bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb, c->partials, s);
+    bh::launch_keys(c->cur, c->n, c->tb, s);
+    MARK(1);
+    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63,
+                         c->n_sys > 1 ? c->tb.bsys : nullptr, c->tb.sort_tmp, c->cap, s);
+    bh::launch_permute(c->cur, c->alt, c->tb, c->n, s);
     std::swap(c->cur, c->alt);
-    HIP_TRY(c, hipEventRecord(ev[2], s));
+    c->parity ^= 1;
+    MARK(2);
     TreeBuffers tb = c->tb;
     if (!mixed) {
         tb.lraw = nullptr;
@@ -186,7 +223,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     }
     bh::launch_build_tree(c->cur, c->n, tb, s);
     if (mixed) {
-        bh::launch_mixed_prepare(c->n, tb, c->mb, G, s);
+        bh::launch_mixed_prepare(c->n, tb, c->mb, c->gp, G, s);
         int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;
         const bool learned = form == BH_FORM_LEARNED;
         if (learned || sched == BH_SCHED_MODEL) {
@@ -197,9 +234,9 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         if (sched == BH_SCHED_MODEL) bh::launch_model_cost(c->n, c->md.feat, c->md, tb, c->mb, s);
         if (sched != BH_SCHED_KEY_ORDER)
             bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, s);
-        HIP_TRY(c, hipEventRecord(ev[3], s));
-        bh::MixedParams mp{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0, c->n_heavy,
-                           sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4};
+        MARK(3);
+        bh::MixedParams mp{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0,
+                           sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4, c->heavy_items};
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
         if (learned && c->trained_steps < c->train_steps) {
             // training phase: the walk's list lengths become samples, then
@@ -210,16 +247,16 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
                                        c->lr, true, nullptr, nullptr, s);
             if (++c->trained_steps >= c->train_steps) c->model_ready = true;
         }
-        HIP_TRY(c, hipEventRecord(ev[4], s));
+        MARK(4);
         HIP_TRY(c, hipGetLastError());
-        c->last_ev = ev;
+        if (ev) c->last_ev = ev;
         c->have_tree = false;   // the plain walk order was not built
         c->have_counts = false;
         c->have_groups = true;
         return BH_OK;
     }
     bh::launch_walk_order(c->n, tb, s);
-    HIP_TRY(c, hipEventRecord(ev[3], s));
+    MARK(3);
     bh::WalkParams p{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0};
     const bool cnt = c->counting;
     if (cnt && !c->cnt_a) {
@@ -230,9 +267,10 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     }
     bh::launch_walk_plain(c->cur, c->n, c->tb, p, cnt ? c->cnt_a : nullptr, cnt ? c->cnt_o : nullptr,
                           cnt ? c->cnt_t : nullptr, s);
-    HIP_TRY(c, hipEventRecord(ev[4], s));
+    MARK(4);
+#undef MARK
     HIP_TRY(c, hipGetLastError());
-    c->last_ev = ev;
+    if (ev) c->last_ev = ev;
     c->have_tree = true;
     c->have_counts = cnt;
     return BH_OK;
@@ -303,6 +341,109 @@ int download3(bh_ctx* c, bh_vec3* out, const double* dx, const double* dy, const
     HIP_TRY(c, hipMemcpyAsync(z.data(), dz, c->n * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < c->n; ++i) out[i] = bh_vec3{x[i], y[i], z[i]};
+    return BH_OK;
+}
+
+// System layout: system s owns bodies [off[s], off[s+1]).  Heavy set of the
+// mixed forms per system: the kMaxHeavy most massive bodies (ties by row),
+// summed directly in FP64; ranks follow row order.  mass is the host copy of
+// the masses in the current body order.
+int configure_systems(bh_ctx* c, const std::vector<int64_t>& counts, const double* mass)
+{
+    const size_t S = counts.size();
+    std::vector<int64_t> off(S + 1, 0);
+    for (size_t k = 0; k < S; ++k) off[k + 1] = off[k] + counts[k];
+    std::vector<bh::SysScalars> sys(S);
+    std::vector<uint8_t> bsys(c->n);
+    std::vector<int8_t> hr(c->n, -1);
+    uint32_t item = 0;
+    for (size_t q = 0; q < S; ++q) {
+        const int64_t lo = off[q], hi = off[q + 1], ns = hi - lo;
+        const int k = static_cast<int>(std::min<int64_t>(kHeavy, ns));
+        std::vector<int64_t> idx(static_cast<size_t>(ns));
+        for (int64_t i = 0; i < ns; ++i) idx[static_cast<size_t>(i)] = lo + i;
+        std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](int64_t a, int64_t b2) {
+            return mass[a] > mass[b2] || (mass[a] == mass[b2] && a < b2);
+        });
+        std::sort(idx.begin(), idx.begin() + k);
+        for (int r = 0; r < k; ++r) hr[static_cast<size_t>(idx[static_cast<size_t>(r)])] = static_cast<int8_t>(r);
+        for (int64_t i = lo; i < hi; ++i) bsys[static_cast<size_t>(i)] = static_cast<uint8_t>(q);
+        bh::SysScalars& y = sys[q];
+        std::memset(&y, 0, sizeof y);
+        y.body_lo = static_cast<uint32_t>(lo);
+        y.body_hi = static_cast<uint32_t>(hi);
+        y.n_heavy = static_cast<uint32_t>(k);
+        y.heavy_chunks = static_cast<uint32_t>((ns + bh::kHeavyChunk - 1) / bh::kHeavyChunk);
+        y.heavy_item = item;
+        item += static_cast<uint32_t>(bh::kMaxHeavy) * y.heavy_chunks;
+    }
+    if (item > c->heavy_item_cap) return fail(c, BH_ERR_STATE, "heavy work items %u > %zu", item, c->heavy_item_cap);
+    hipStream_t st = c->stream;
+    HIP_TRY(c, hipMemcpyAsync(c->cur.hrank, hr.data(), c->n, hipMemcpyHostToDevice, st));
+    HIP_TRY(c, hipMemcpyAsync(c->bsys_w, bsys.data(), c->n, hipMemcpyHostToDevice, st));
+    HIP_TRY(c, hipMemcpyAsync(c->tb.sys, sys.data(), S * sizeof(bh::SysScalars), hipMemcpyHostToDevice, st));
+    HIP_TRY(c, hipStreamSynchronize(st));
+    c->n_sys = static_cast<uint32_t>(S);
+    c->tb.n_sys = c->n_sys;
+    c->sys_off = off;
+    c->heavy_items = item;
+    drop_graphs(c);
+    return BH_OK;
+}
+
+// Mixed forms after the training phase of the learned schedule: the same
+// launches every step, so bh_step can replay a captured graph.
+bool steady_step(const bh_ctx* c)
+{
+    if (c->t <= 0.0) return false;   // the first step has no leading kick
+    if (c->cfg.form == BH_FORM_LEARNED && c->trained_steps < c->train_steps) return false;
+    if (c->cfg.form == BH_FORM_PLAIN && c->counting) return false;
+    return true;
+}
+
+std::vector<double> graph_key(const bh_ctx* c, double dt)
+{
+    return {static_cast<double>(c->cfg.form), c->cfg.theta, c->cfg.G, c->cfg.soft2, dt, static_cast<double>(c->n),
+            static_cast<double>(c->n_sys), static_cast<double>(c->sched), c->model_ready ? 1.0 : 0.0,
+            static_cast<double>(c->gp.gmax), c->gp.efac, static_cast<double>(c->gp.occ),
+            static_cast<double>(c->mixed_blocks), static_cast<double>(c->cap)};
+}
+
+// One steady step through the graph of the current parity (captured on first use).
+int graph_step(bh_ctx* c, double dt)
+{
+    const int p = c->parity;
+    const std::vector<double> key = graph_key(c, dt);
+    if (!c->gexec[p] || c->gkey[p] != key) {
+        if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+        c->gexec[p] = nullptr;
+        hipGraph_t g = nullptr;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = run_force(c, c->cfg.form, c->cfg.theta, c->cfg.G, c->cfg.soft2, true, true, dt, true, nullptr);
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(c, BH_ERR_HIP, "stream capture: %s", hipGetErrorString(e));
+        // run_force swapped the buffers while it was captured; undo, the replay does the work
+        std::swap(c->cur, c->alt);
+        c->parity ^= 1;
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->gexec[p] = nullptr;
+            return fail(c, BH_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
+        c->gkey[p] = key;
+    }
+    HIP_TRY(c, hipGraphLaunch(c->gexec[p], c->stream));
+    std::swap(c->cur, c->alt);
+    c->parity ^= 1;
+    c->have_tree = c->cfg.form == BH_FORM_PLAIN;
+    c->have_counts = false;
+    c->have_groups = c->cfg.form != BH_FORM_PLAIN;
+    ++c->graph_replays;
     return BH_OK;
 }
 
@@ -394,10 +535,19 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
     if (rc) return rc;
     c->n = static_cast<size_t>(n);
     c->t = 0.0;
-    if (c->mixed_blocks == 0) c->mixed_blocks = bh::mixed_walk_blocks(c->cfg.device);
+    c->parity = 0;
+    drop_graphs(c);
+    if (c->mixed_blocks_max == 0) c->mixed_blocks_max = bh::mixed_walk_blocks(c->cfg.device);
+    if (c->mixed_blocks == 0) c->mixed_blocks = c->mixed_blocks_max;
     c->have_tree = false;
     c->have_counts = false;
-    if (n == 0) return BH_OK;
+    c->have_groups = false;
+    if (n == 0) {
+        c->n_sys = 1;
+        c->tb.n_sys = 1;
+        c->sys_off.assign({0, 0});
+        return BH_OK;
+    }
     std::vector<double> x, y, z;
     hipStream_t s = c->stream;
     to_soa(pos, c->n, x, y, z);
@@ -417,23 +567,9 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
         ids = idv.data();
     }
     HIP_TRY(c, hipMemcpyAsync(c->cur.id, ids, n * 8, hipMemcpyHostToDevice, s));
-    {
-        // heavy set of the mixed forms: the kHeavy most massive bodies (ties by
-        // row), summed directly in FP64; ranks follow row order
-        const int k = static_cast<int>(std::min<int64_t>(kHeavy, n));
-        std::vector<int64_t> idx(c->n);
-        for (size_t i = 0; i < c->n; ++i) idx[i] = static_cast<int64_t>(i);
-        std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](int64_t a, int64_t b2) {
-            return mass[a] > mass[b2] || (mass[a] == mass[b2] && a < b2);
-        });
-        std::sort(idx.begin(), idx.begin() + k);
-        std::vector<int8_t> hr(c->n, -1);
-        for (int r = 0; r < k; ++r) hr[idx[r]] = static_cast<int8_t>(r);
-        c->n_heavy = k;
-        HIP_TRY(c, hipMemcpyAsync(c->cur.hrank, hr.data(), n, hipMemcpyHostToDevice, s));
-        HIP_TRY(c, hipMemsetAsync(c->cur.pcost, 0, n * sizeof(uint32_t), s));
-        HIP_TRY(c, hipStreamSynchronize(s));
-    }
+    HIP_TRY(c, hipMemsetAsync(c->cur.pcost, 0, n * sizeof(uint32_t), s));
+    rc = configure_systems(c, std::vector<int64_t>{n}, mass);
+    if (rc) return rc;
     HIP_TRY(c, hipMemsetAsync(c->cur.ax, 0, n * 8, s));
     HIP_TRY(c, hipMemsetAsync(c->cur.ay, 0, n * 8, s));
     HIP_TRY(c, hipMemsetAsync(c->cur.az, 0, n * 8, s));
@@ -480,11 +616,16 @@ int bh_step(bh_ctx* c, int64_t nsteps, double dt)
     }
     c->ring_steps = 0;
     for (int64_t k = 0; k < nsteps; ++k) {
-        const hipEvent_t* ev = c->ring.data() + 5 * (k % kRing);
-        int rc = run_force(c, c->cfg.form, c->cfg.theta, c->cfg.G, c->cfg.soft2, c->t > 0, true, dt, true, ev);
+        int rc;
+        if (c->use_graph && steady_step(c)) {
+            rc = graph_step(c, dt);
+        } else {
+            const hipEvent_t* ev = c->ring.data() + 5 * (c->ring_steps % kRing);
+            rc = run_force(c, c->cfg.form, c->cfg.theta, c->cfg.G, c->cfg.soft2, c->t > 0, true, dt, true, ev);
+            if (rc == BH_OK) ++c->ring_steps;
+        }
         if (rc) return rc;
         c->t += dt;
-        c->ring_steps = k + 1;
     }
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return check_device_errors(c);
@@ -530,10 +671,10 @@ int bh_export_bbox(bh_ctx* c, double* bb6)
 {
     if (check_ctx(c) || !bb6) return BH_ERR_ARG;
     if (!c->have_tree) return fail(c, BH_ERR_STATE, "no force evaluation yet");
-    StepScalars sc;
+    bh::SysScalars y;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
-    std::memcpy(bb6, sc.bb, sizeof sc.bb);
+    HIP_TRY(c, hipMemcpy(&y, c->tb.sys, sizeof y, hipMemcpyDeviceToHost));
+    std::memcpy(bb6, y.bb, sizeof y.bb);
     return BH_OK;
 }
 
@@ -678,7 +819,7 @@ int bh_get_phase_ms(bh_ctx* c, double* ms5)
 int bh_get_step_phase_ms(bh_ctx* c, double* ms5, int64_t* nsteps)
 {
     if (check_ctx(c) || !ms5) return BH_ERR_ARG;
-    if (c->ring_steps == 0) return fail(c, BH_ERR_STATE, "no bh_step call yet");
+    if (c->ring_steps == 0) return fail(c, BH_ERR_STATE, "no eager step in the last bh_step call (graph steps record no phases)");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     const int64_t k0 = c->ring_steps > kRing ? c->ring_steps - kRing : 0;
     double acc[5] = {0, 0, 0, 0, 0};
@@ -699,6 +840,149 @@ int bh_set_schedule(bh_ctx* c, int mode)
     if (check_ctx(c)) return BH_ERR_ARG;
     if (mode < BH_SCHED_KEY_ORDER || mode > BH_SCHED_MODEL) return fail(c, BH_ERR_ARG, "unknown schedule %d", mode);
     c->sched = mode;
+    return BH_OK;
+}
+
+int bh_set_graph(bh_ctx* c, int on)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->use_graph = on != 0;
+    if (!c->use_graph) drop_graphs(c);
+    return BH_OK;
+}
+
+int bh_get_graph_replays(bh_ctx* c, int64_t* replays)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (replays) *replays = c->graph_replays;
+    return BH_OK;
+}
+
+int bh_set_systems(bh_ctx* c, int64_t n_systems, const int64_t* counts)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (n_systems < 1 || n_systems > bh::kMaxSystems || !counts)
+        return fail(c, BH_ERR_ARG, "1..%d systems", bh::kMaxSystems);
+    if (c->n == 0) return fail(c, BH_ERR_STATE, "no bodies (bh_set_bodies first)");
+    int64_t sum = 0;
+    for (int64_t k = 0; k < n_systems; ++k) {
+        if (counts[k] < 1) return fail(c, BH_ERR_ARG, "system %lld has %lld bodies", (long long)k, (long long)counts[k]);
+        sum += counts[k];
+    }
+    if (sum != static_cast<int64_t>(c->n))
+        return fail(c, BH_ERR_ARG, "system sizes add to %lld, not %zu", (long long)sum, c->n);
+    std::vector<double> mass(c->n);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(mass.data(), c->cur.m, c->n * 8, hipMemcpyDeviceToHost));
+    c->have_tree = c->have_counts = c->have_groups = false;
+    return configure_systems(c, std::vector<int64_t>(counts, counts + n_systems), mass.data());
+}
+
+int bh_get_systems(bh_ctx* c, int64_t* n_systems, int64_t capacity, int64_t* offsets)
+{
+    if (check_ctx(c) || !n_systems) return BH_ERR_ARG;
+    *n_systems = c->n == 0 ? 0 : c->n_sys;
+    if (!offsets || c->n == 0) return BH_OK;
+    if (capacity < static_cast<int64_t>(c->sys_off.size())) return fail(c, BH_ERR_OVERFLOW, "%zu offsets", c->sys_off.size());
+    std::copy(c->sys_off.begin(), c->sys_off.end(), offsets);
+    return BH_OK;
+}
+
+int bh_set_group_params(bh_ctx* c, int gmax, double efac, int occ, int walk_blocks)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (gmax < 1 || gmax > bh::kGroupMax || !(efac > 0.0) || occ < 1 || walk_blocks < 0)
+        return fail(c, BH_ERR_ARG, "bad group parameters");
+    c->gp.gmax = static_cast<uint32_t>(gmax);
+    c->gp.efac = efac;
+    c->gp.occ = static_cast<uint32_t>(occ);
+    if (c->mixed_blocks_max == 0) c->mixed_blocks_max = bh::mixed_walk_blocks(c->cfg.device);
+    c->mixed_blocks = walk_blocks == 0 ? c->mixed_blocks_max : std::min<unsigned>(walk_blocks, c->mixed_blocks_max);
+    return BH_OK;
+}
+
+int bh_get_group_params(bh_ctx* c, int* gmax, double* efac, int* occ, int* walk_blocks)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (gmax) *gmax = static_cast<int>(c->gp.gmax);
+    if (efac) *efac = c->gp.efac;
+    if (occ) *occ = static_cast<int>(c->gp.occ);
+    if (walk_blocks) *walk_blocks = static_cast<int>(c->mixed_blocks);
+    return BH_OK;
+}
+
+int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (form != BH_FORM_MIXED && form != BH_FORM_LEARNED)
+        return fail(c, BH_ERR_ARG, "the autotuner tunes the mixed forms");
+    if (c->n == 0) return fail(c, BH_ERR_STATE, "no bodies");
+    if (reps < 1) reps = 1;
+    const int saved_sched = c->sched;
+    int tform = form;
+    if (form == BH_FORM_LEARNED && !c->model_ready) {
+        tform = BH_FORM_MIXED;
+        c->sched = BH_SCHED_PREV_COST;
+    }
+    if (c->mixed_blocks_max == 0) c->mixed_blocks_max = bh::mixed_walk_blocks(c->cfg.device);
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    auto time_one = [&](double* ms) -> int {
+        std::vector<double> t;
+        for (int r = 0; r <= reps; ++r) {   // r == 0 warms up (and records the costs the schedule uses)
+            HIP_TRY(c, hipEventRecord(e0, c->stream));
+            int rc = run_force(c, tform, c->cfg.theta, c->cfg.G, c->cfg.soft2, false, false, 0.0, false, nullptr);
+            if (rc) return rc;
+            HIP_TRY(c, hipEventRecord(e1, c->stream));
+            HIP_TRY(c, hipEventSynchronize(e1));
+            if ((rc = check_device_errors(c))) return rc;
+            float f = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&f, e0, e1));
+            if (r > 0) t.push_back(f);
+        }
+        std::sort(t.begin(), t.end());
+        *ms = t[t.size() / 2];
+        return BH_OK;
+    };
+    const bh::GroupParams def{};
+    bh::GroupParams best = def;
+    unsigned best_blocks = c->mixed_blocks_max;
+    double best_ms = 0.0, def_ms = 0.0;
+    int tried = 0, rc = BH_OK;
+    const uint32_t gmaxes[3] = {32, 48, 64};
+    const double efacs[3] = {1.0, 2.0, 4.0};
+    const unsigned blocks[2] = {c->mixed_blocks_max, std::max(1u, c->mixed_blocks_max / 2)};
+    for (unsigned bl : blocks)
+        for (uint32_t gm : gmaxes)
+            for (double ef : efacs) {
+                c->gp.gmax = gm;
+                c->gp.efac = ef;
+                c->gp.occ = def.occ;
+                c->mixed_blocks = bl;
+                double ms = 0.0;
+                if ((rc = time_one(&ms))) break;
+                ++tried;
+                if (gm == def.gmax && ef == def.efac && bl == c->mixed_blocks_max) def_ms = ms;
+                if (tried == 1 || ms < best_ms) {
+                    best_ms = ms;
+                    best = c->gp;
+                    best_blocks = bl;
+                }
+            }
+    c->sched = saved_sched;
+    c->gp = best;
+    c->mixed_blocks = best_blocks;
+    if (rc) return rc;
+    if (out) {
+        out->gmax = static_cast<int>(best.gmax);
+        out->efac = best.efac;
+        out->occ = static_cast<int>(best.occ);
+        out->walk_blocks = static_cast<int>(best_blocks);
+        out->best_ms = best_ms;
+        out->default_ms = def_ms;
+        out->n_tried = tried;
+    }
     return BH_OK;
 }
 
@@ -754,7 +1038,7 @@ int bh_direct_sum(bh_ctx* c, int64_t m, const int64_t* idx, double G, double sof
     int rc = BH_OK;
     e = hipMemcpyAsync(d_idx, idx, m * 8, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        bh::launch_direct_sum(c->cur, c->n, d_idx, static_cast<size_t>(m), G, soft2, d_acc, c->stream);
+        bh::launch_direct_sum(c->cur, c->tb, d_idx, static_cast<size_t>(m), G, soft2, d_acc, c->stream);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(acc_out, d_acc, m * 24, hipMemcpyDeviceToHost, c->stream);

@@ -1,5 +1,6 @@
 // bh_direct.hip -- FP64 direct-sum accelerations for sample bodies, the
-// error reference for every walk form.  One block per sample body; each
+// error reference for every walk form.  A sample sees the bodies of its own
+// system only (systems are independent).  One block per sample body; each
 // thread runs a Neumaier-compensated sum over a strided share of the bodies,
 // and thread 0 combines the 256 compensated partials in thread order, so the
 // result is deterministic for a given launch shape.
@@ -26,13 +27,15 @@ struct Neumaier {
     __device__ double value() const { return s + c; }
 };
 
-__global__ __launch_bounds__(kNT) void k_direct_sum(Bodies b, size_t n, const int64_t* __restrict__ idx, double G,
-                                                    double soft2, double* __restrict__ acc3)
+__global__ __launch_bounds__(kNT) void k_direct_sum(Bodies b, TreeBuffers tb, const int64_t* __restrict__ idx,
+                                                    double G, double soft2, double* __restrict__ acc3)
 {
     const int64_t i = idx[blockIdx.x];
     const double px = b.px[i], py = b.py[i], pz = b.pz[i];
+    const SysScalars* sy = tb.sys + tb.bsys[i];
+    const size_t j0 = sy->body_lo, j1 = sy->body_hi;
     Neumaier ax, ay, az;
-    for (size_t j = threadIdx.x; j < n; j += kNT) {
+    for (size_t j = j0 + threadIdx.x; j < j1; j += kNT) {
         if (static_cast<int64_t>(j) == i) continue;
         const double dx = b.px[j] - px, dy = b.py[j] - py, dz = b.pz[j] - pz;
         const double r2 = dx * dx + dy * dy + dz * dz + soft2;
@@ -57,11 +60,11 @@ __global__ __launch_bounds__(kNT) void k_direct_sum(Bodies b, size_t n, const in
 
 }  // namespace
 
-void launch_direct_sum(const Bodies& b, size_t n, const int64_t* idx, size_t m, double G, double soft2,
+void launch_direct_sum(const Bodies& b, const TreeBuffers& tb, const int64_t* idx, size_t m, double G, double soft2,
                        double* acc3, hipStream_t s)
 {
     if (m == 0) return;
-    k_direct_sum<<<static_cast<unsigned>(m), kNT, 0, s>>>(b, n, idx, G, soft2, acc3);
+    k_direct_sum<<<static_cast<unsigned>(m), kNT, 0, s>>>(b, tb, idx, G, soft2, acc3);
 }
 
 }  // namespace bh

@@ -10,6 +10,11 @@
 //           compressed node in walk order (pre-order, children visited from
 //           octant 7 down to 0) with a skip index to the node that follows its
 //           subtree.
+//   systems independent systems share one step (the ensemble form): system s
+//           owns the bodies [body_lo, body_hi) for good (sorting permutes
+//           inside a system only), a per-body system id, and a SysScalars
+//           record with its box, key spans, side lengths, root, leaf range
+//           and heavy set.  One system is the special case S = 1.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,15 +44,33 @@ constexpr uint32_t kMetaTopShift = 5;         // chain top depth (5 bits)
 constexpr uint32_t kMetaLeaf = 1u << 10;      // distinct-key leaf (depth 21)
 constexpr uint32_t kMetaChain = 1u << 11;     // leaf chain starts above depth 21
 
-// Device-side scalars shared by the kernels of one force evaluation.
-struct StepScalars {
+constexpr int kMaxHeavy = 16;     // bodies per system summed directly in FP64 by the mixed forms
+constexpr int kMaxSystems = 256;  // systems per context (the sort's system pass uses 8-bit digits)
+constexpr uint8_t kNoSplit = 0xff;   // delta of a boundary between two systems
+
+// Per-system scalars.  Fixed by the system layout: body range, heavy count
+// and heavy work offsets.  Per step: everything else.
+struct SysScalars {
     double bb[6];        // box min xyz, max xyz
     double span[3];      // per-axis normalisation span (1.0 when degenerate)
-    double side2[22];    // (L / 2^d)^2, L = largest box extent
+    double side2[kMaxDepth + 1];   // (L / 2^d)^2, L = largest box extent
+    double emax;         // walk-group extent limit of the mixed forms
+    uint32_t body_lo, body_hi;
+    uint32_t leaf_lo;    // first leaf of the system (leaves are in system order)
+    uint32_t root;       // unified id of the system's root (leaf ids 0..M-1, branch M..)
+    uint32_t n_heavy;
+    uint32_t heavy_item; // first work item of the system's heavy direct sums
+    uint32_t heavy_chunks;   // 1024-body chunks of the system (heavy direct sums)
+    uint32_t heavy_slot[kMaxHeavy];   // sorted position of each heavy body, by rank
+    uint32_t dcount[kMaxDepth + 1];   // boundaries splitting at each depth (mixed forms)
+};
+
+// Device-side scalars shared by the kernels of one force evaluation.
+struct StepScalars {
     uint32_t n_leaves;   // M: distinct keys
     uint32_t n_branch;   // B: branching nodes
     uint32_t n_nodes;    // M + B
-    uint32_t root;       // unified id of the root (leaf ids 0..M-1, branch M..)
+    uint32_t pad0;
     uint64_t n_full;     // node count of the reference's full-chain map
     uint32_t block_counter;  // last-block-done counter, left at 0
     uint32_t error;          // kErr* bits raised by bounds guards
@@ -79,8 +102,10 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size
 // scan_error_word / sort_error_word: device address of the word that becomes
 // nonzero if a look-back wait of a scan / sort using that tmp ever gave up.
 size_t sort_tmp_words(size_t cap);
+// With sysd (the system of each index; values < 256) a last stable pass
+// orders by sysd[payload], i.e. by (system, key).
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* tmp, size_t cap, hipStream_t s);
+                      const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s);
 uint32_t* scan_error_word(uint32_t* scan_tmp);
 uint32_t* sort_error_word(uint32_t* sort_tmp);
 
@@ -121,23 +146,27 @@ struct TreeBuffers {
     double* raw;          // 4 per node: mass, sum m*x, sum m*y, sum m*z
     double* lraw;         // the same over light bodies only (mixed forms), or null
     double* lbox;         // 6 per node: bounding box of the light bodies (min xyz, max xyz)
-    uint32_t* heavy_slot; // sorted position of each heavy body, by rank
     uint32_t* gflag;      // walk-group start flags (mixed forms), cleared per step
     WalkNode* walk;       // capacity 2N
     uint32_t* scan_tmp;
     uint32_t* sort_tmp;
     StepScalars* sc;
+    // systems
+    SysScalars* sys;      // n_sys records
+    const uint8_t* bsys;  // system of each body position (fixed; all 0 for one system)
+    uint32_t n_sys;
 };
 
-// kick (optional) + drift + box; writes sc->bb/span/side2.
+// kick (optional) + drift + per-system box; writes bb/span/side2 of every
+// system.  partials hold 6 doubles per block (at most kBoxBlocks blocks).
+constexpr unsigned kBoxBlocks = 1024;
 void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
-                           StepScalars* sc, double* partials, hipStream_t s);
-// Morton keys (src/morton_keys.cpp:65-98) + identity payload.
-void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key, uint32_t* perm,
-                 hipStream_t s);
-// Gather every body array through perm into dst; first-of-key flags.
-void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
-                    uint32_t* flag, uint32_t* heavy_slot, size_t n, hipStream_t s);
+                           const TreeBuffers& tb, double* partials, hipStream_t s);
+// Morton keys (src/morton_keys.cpp:65-98) in each body's system + identity payload.
+void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s);
+// Gather every body array through perm into dst; first-of-key flags (a new
+// system also starts a leaf); heavy slots of every system.
+void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s);
 // Tree topology and raw moments from sorted keys / bodies and first-of-key
 // flags (tb.flag); light moments too when tb.lraw is set.
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s);
@@ -148,6 +177,7 @@ struct WalkParams {
     double theta2, G, soft2, half_dt;
     int kick;   // apply v += half_dt * a after the walk
 };
+// One system only (the plain form runs a single system).
 void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const WalkParams& p,
                        int64_t* cnt_accept, int64_t* cnt_open, int64_t* cnt_test, hipStream_t s);
 
@@ -166,9 +196,8 @@ struct MNode {
 };
 static_assert(sizeof(MNode) == 48, "MNode is 48 bytes");
 constexpr uint32_t kMLeaf = 1u << 31;
-constexpr int kGroupMax = 64;   // bodies per walk group (one wavefront)
-constexpr int kMaxHeavy = 16;   // bodies summed directly in FP64 by the mixed forms
-constexpr int kHeavyChunks = 64;
+constexpr int kGroupMax = 64;   // bodies per walk group (one wavefront): the largest group limit
+constexpr uint32_t kHeavyChunk = 1024;   // bodies per work item of the heavy direct sums
 
 struct MixedBuffers {
     MNode* node;        // 2N
@@ -185,9 +214,7 @@ struct MixedBuffers {
     uint32_t* order;    // N: group order of the scheduled walk
     uint32_t* chunk;    // workers + 1: static chunk starts into order[]
     float* gcost;       // N: estimated cost of each group
-    double* hpart;      // heavy direct sums: kMaxHeavy x kHeavyChunks x 3 partials
-    uint32_t* dhist;    // 22: boundaries splitting at each depth
-    uint32_t* dhist_part;  // per-block counts behind dhist
+    double* hpart;      // heavy direct sums: 3 partials per work item (system, heavy rank, chunk)
     uint32_t* sched_hist;    // 64 cost buckets (left at zero)
     uint32_t* sched_cursor;  // 64
 };
@@ -195,13 +222,23 @@ struct MixedBuffers {
 struct MixedParams {
     double theta2, G, soft2, half_dt;
     int kick;
-    int n_heavy;
     int sched;            // 0 dynamic in key order, 1 dynamic over order[], 2 static chunks
     uint32_t n_workers;
+    uint32_t heavy_items; // work items of the heavy direct sums over all systems
+};
+
+// Walk-group shape, the tunables of the mixed walk: at most gmax (<= 64)
+// bodies per group, and a group box no wider than efac x the cell side at the
+// first depth whose non-empty cells number at least (system size) / occ.
+struct GroupParams {
+    uint32_t gmax = 64;
+    double efac = 2.0;
+    uint32_t occ = 32;
 };
 
 // Walk groups and the children-contiguous node array (after the tree build).
-void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s);
+void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, double G,
+                          hipStream_t s);
 // Persistent mixed walk over the groups; returns the number of worker waves.
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
                        const MixedParams& p, unsigned blocks, hipStream_t s);
@@ -244,8 +281,9 @@ void launch_model_train(const ModelBuffers& md, const float* x, const float* t, 
                         hipStream_t s);
 void launch_refresh_bf16(const ModelBuffers& md, hipStream_t s);
 
-// FP64 direct sum for m sample bodies (Neumaier-compensated).
-void launch_direct_sum(const Bodies& b, size_t n, const int64_t* idx, size_t m, double G, double soft2,
+// FP64 direct sum for m sample bodies over the bodies of each sample's system
+// (Neumaier-compensated).
+void launch_direct_sum(const Bodies& b, const TreeBuffers& tb, const int64_t* idx, size_t m, double G, double soft2,
                        double* acc3, hipStream_t s);
 
 }  // namespace bh

@@ -35,6 +35,7 @@
 // over all bodies (k_heavy_direct).  The sum of the two parts is FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "bh_internal.hpp"
@@ -70,44 +71,64 @@ __device__ inline uint32_t unit_first_body(const TreeBuffers& tb, uint32_t M, ui
     return t < M ? tb.leaf_first[t] : tb.leaf_first[tb.blo[t - M]];
 }
 
-// Cell occupancy by depth: hist[d] = number of boundaries u[j] | u[j+1] that
-// split at depth d, so depth d holds 1 + sum_{e < d} hist[e] non-empty cells.
-// Each wavefront counts 64 boundaries per round with one ballot per depth
-// (no atomics in the loop); every block leaves its 22 counts in
-// mb.dhist_part and k_split_reduce adds them in block order.
+// Cell occupancy by depth, per system: sys[s].dcount[d] = number of
+// boundaries u[j] | u[j+1] inside system s that split at depth d, so depth d
+// holds 1 + sum_{e < d} dcount[e] non-empty cells of s.  With one system each
+// wavefront counts 64 boundaries per round with one ballot per depth; with
+// several, LDS counters per (system, depth).  Blocks add their counts into
+// dcount, which k_sys_extent reads and clears.
 constexpr unsigned kHistBlocks = 128;
 
-__global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb, MixedBuffers mb)
+__global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
 {
-    __shared__ uint32_t h[kWaves][kMaxDepth + 1];
+    __shared__ uint32_t h[kMaxSystems][kMaxDepth + 1];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t M = tb.sc->n_leaves;
-    uint32_t cnt[kMaxDepth + 1];
-#pragma unroll
-    for (int d = 0; d <= kMaxDepth; ++d) cnt[d] = 0;
-    const uint32_t stride = gridDim.x * kNT;
-    for (uint32_t j0 = blockIdx.x * kNT + w * 64; j0 + 1 < M; j0 += stride) {
-        const uint32_t j = j0 + static_cast<uint32_t>(lane);
-        const int d = (j + 1 < M) ? static_cast<int>(tb.delta[j]) : -1;
-#pragma unroll
-        for (int e = 0; e <= kMaxDepth; ++e) cnt[e] += static_cast<uint32_t>(__popcll(__ballot(d == e)));
-    }
-    if (lane == 0)
-        for (int d = 0; d <= kMaxDepth; ++d) h[w][d] = cnt[d];
+    const uint32_t rows = tb.n_sys;
+    for (uint32_t k = threadIdx.x; k < rows * (kMaxDepth + 1); k += kNT) (&h[0][0])[k] = 0;
     __syncthreads();
-    if (threadIdx.x <= kMaxDepth) {
-        uint32_t s = 0;
-        for (int k = 0; k < kWaves; ++k) s += h[k][threadIdx.x];
-        mb.dhist_part[blockIdx.x * (kMaxDepth + 1) + threadIdx.x] = s;
+    const uint32_t stride = gridDim.x * kNT;
+    if (rows == 1) {
+        uint32_t cnt[kMaxDepth + 1];
+#pragma unroll
+        for (int d = 0; d <= kMaxDepth; ++d) cnt[d] = 0;
+        for (uint32_t j0 = blockIdx.x * kNT + w * 64; j0 + 1 < M; j0 += stride) {
+            const uint32_t j = j0 + static_cast<uint32_t>(lane);
+            const int d = (j + 1 < M) ? static_cast<int>(tb.delta[j]) : -1;
+#pragma unroll
+            for (int e = 0; e <= kMaxDepth; ++e) cnt[e] += static_cast<uint32_t>(__popcll(__ballot(d == e)));
+        }
+        if (lane == 0)
+            for (int d = 0; d <= kMaxDepth; ++d) atomicAdd(&h[0][d], cnt[d]);
+    } else {
+        for (uint32_t j = blockIdx.x * kNT + threadIdx.x; j + 1 < M; j += stride) {
+            const uint8_t d = tb.delta[j];
+            if (d != kNoSplit) atomicAdd(&h[tb.bsys[tb.leaf_first[j]]][d], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < rows * (kMaxDepth + 1); k += kNT) {
+        const uint32_t v = (&h[0][0])[k];
+        if (v) atomicAdd(&tb.sys[k / (kMaxDepth + 1)].dcount[k % (kMaxDepth + 1)], v);
     }
 }
 
-__global__ __launch_bounds__(64) void k_split_reduce(MixedBuffers mb)
+// Largest box extent a walk group of system s may have: efac x the cell side
+// at the first depth whose non-empty cells number at least n_s / occ (with
+// occ = 32, cells there hold ~32 bodies on average).  A group much wider than
+// its neighbours' spacing would force the walk to open everything its box
+// overlaps.  Clears dcount for the next step.
+__global__ __launch_bounds__(kNT) void k_sys_extent(TreeBuffers tb, GroupParams gp)
 {
-    if (threadIdx.x > kMaxDepth) return;
-    uint32_t s = 0;
-    for (unsigned b = 0; b < kHistBlocks; ++b) s += mb.dhist_part[b * (kMaxDepth + 1) + threadIdx.x];
-    mb.dhist[threadIdx.x] = s;
+    for (uint32_t q = threadIdx.x; q < tb.n_sys; q += kNT) {
+        SysScalars* sy = tb.sys + q;
+        const size_t ns = sy->body_hi - sy->body_lo;
+        uint32_t cells = 1;
+        int d = 0;
+        for (; d < kMaxDepth && static_cast<size_t>(cells) * gp.occ < ns; ++d) cells += sy->dcount[d];
+        sy->emax = gp.efac * sqrt(sy->side2[d]);
+        for (int e = 0; e <= kMaxDepth; ++e) sy->dcount[e] = 0;
+    }
 }
 
 __device__ inline double box_extent(const double* bx)
@@ -116,25 +137,14 @@ __device__ inline double box_extent(const double* bx)
     return e > 0.0 ? e : 0.0;   // an empty box (no light bodies) has extent 0
 }
 
-// Largest box extent a walk group may have: twice the cell side at the first
-// depth whose non-empty cells number at least n/32 (cells there hold ~32
-// bodies on average).  A group much wider than its neighbours' spacing would
-// force the walk to open everything its box overlaps.
-__device__ inline double group_extent_limit(const TreeBuffers& tb, const MixedBuffers& mb, size_t n)
-{
-    uint32_t cells = 1;
-    int d = 0;
-    for (; d < kMaxDepth && static_cast<size_t>(cells) * 32 < n; ++d) cells += mb.dhist[d];
-    return 2.0 * sqrt(tb.sc->side2[d]);
-}
-
 // group starts (flags at the first body of every group).  A unit is "good"
-// when it holds <= 64 bodies and its light bodies' box is within the extent
-// limit (a leaf always qualifies); goodness passes to children.  Groups are
-// the maximal good units, and runs of good siblings are packed while the
-// count fits 64 and their joint box stays within the limit.  Everything here
-// depends on the bodies alone, so every schedule sees the same groups.
-__global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, MixedBuffers mb)
+// when it holds <= gmax bodies and its light bodies' box is within its
+// system's extent limit (a leaf always qualifies); goodness passes to
+// children.  Groups are the maximal good units, and runs of good siblings are
+// packed while the count fits gmax and their joint box stays within the
+// limit.  Everything here depends on the system's bodies alone, so every
+// schedule, and every other system of an ensemble, sees the same groups.
+__global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, MixedBuffers mb, uint32_t gmax)
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
@@ -142,9 +152,9 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     const uint32_t u = static_cast<uint32_t>(t);
     const uint32_t cnt = unit_count(tb, M, u);
     const uint32_t depth = u < M ? kMaxDepth : tb.bdepth[u - M];
-    const double emax = group_extent_limit(tb, mb, n);
+    const double emax = tb.sys[tb.bsys[unit_first_body(tb, M, u)]].emax;
     auto good = [&](uint32_t v, uint32_t cv) {
-        return cv <= static_cast<uint32_t>(kGroupMax) && (v < M || box_extent(tb.lbox + 6 * static_cast<size_t>(v)) <= emax);
+        return cv <= gmax && (v < M || box_extent(tb.lbox + 6 * static_cast<size_t>(v)) <= emax);
     };
     if (tb.parent[u] == kNone && good(u, cnt)) {
         const uint32_t f = unit_first_body(tb, M, u);
@@ -153,9 +163,9 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
         return;
     }
     if (u < M) {
-        if (cnt > kGroupMax) {   // co-keyed bodies beyond one group
+        if (cnt > gmax) {   // co-keyed bodies beyond one group
             const uint32_t f = tb.leaf_first[u];
-            for (uint32_t k = 0; k < cnt; k += kGroupMax) {
+            for (uint32_t k = 0; k < cnt; k += gmax) {
                 tb.gflag[f + k] = 1;
                 mb.gdepth[f + k] = kMaxDepth;
             }
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
         }
         const double* bx = tb.lbox + 6 * static_cast<size_t>(cv);
         const bool empty = !(bx[0] <= bx[3]);
-        bool fits = open && cur + cc <= static_cast<uint32_t>(kGroupMax);
+        bool fits = open && cur + cc <= gmax;
         if (fits && !empty) {
             double e = 0.0;
             for (int a = 0; a < 3; ++a) e = fmax(e, fmax(ru[a], bx[3 + a]) - fmin(rl[a], bx[a]));
@@ -226,15 +236,16 @@ __global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, u
     out[bi] = c;
 }
 
-// slots: root 0; children of branching node b at 1 + base[b] + rank
+// slots: the root of system s at s; children of branching node b at
+// n_sys + base[b] + rank
 __global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb)
 {
     const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (bi == 0) mb.slot[tb.sc->root] = 0;
+    if (bi < tb.n_sys) mb.slot[tb.sys[bi].root] = static_cast<uint32_t>(bi);
     if (bi >= tb.sc->n_branch) return;
     const int32_t* ch = tb.child + 8 * bi;
     uint32_t k = 0;
-    const uint32_t b0 = 1u + mb.base[bi];
+    const uint32_t b0 = tb.n_sys + mb.base[bi];
 #pragma unroll
     for (int o = 0; o < 8; ++o)
         if (ch[o] >= 0) mb.slot[ch[o]] = b0 + k++;
@@ -278,7 +289,7 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, TreeBuffers tb, M
         const int32_t* ch = tb.child + 8 * static_cast<size_t>(bi);
 #pragma unroll
         for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
-        nd.first = 1u + mb.base[bi];
+        nd.first = tb.n_sys + mb.base[bi];
         nd.info = c;
     }
     mb.node[mb.slot[t]] = nd;
@@ -434,7 +445,8 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         Acc3 acc;
         uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0;
         bool overflow = false;
-        if (lane == 0) stk[0] = enc(0, 1, false);
+        const uint32_t gsys = tb.bsys[b0];
+        if (lane == 0) stk[0] = enc(gsys, 1, false);   // the root of the group's system
         __builtin_amdgcn_wave_barrier();
         while (sp > 0 && !overflow) {
             // pop up to 64 ranges, one per lane, while the stack cannot overflow
@@ -548,10 +560,12 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         __builtin_amdgcn_wave_barrier();
         flush_list(list, ln, ox, oy, oz, soft2f, acc);
         __builtin_amdgcn_wave_barrier();
-        // heavy bodies: FP64 direct sum in rank order
+        // heavy bodies of the system: FP64 direct sum in rank order
         double hax = 0.0, hay = 0.0, haz = 0.0;
-        for (int k = 0; k < p.n_heavy; ++k) {
-            const uint32_t j = tb.heavy_slot[k];
+        const SysScalars* sy = tb.sys + gsys;
+        const uint32_t nh = sy->n_heavy;
+        for (uint32_t k = 0; k < nh; ++k) {
+            const uint32_t j = sy->heavy_slot[k];
             if (j == i) continue;
             const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
             const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
@@ -603,51 +617,74 @@ struct Neumaier {
     __device__ double value() const { return s + c; }
 };
 
-// Block (k, c) sums heavy body k over the c-th of kHeavyChunks body ranges;
-// k_heavy_final adds the chunk partials in chunk order.
-__global__ __launch_bounds__(kNT) void k_heavy_partial(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
-                                                       MixedParams p)
+// Work item w = (system s, heavy rank k, chunk c): heavy body k of system s
+// summed over the c-th run of 1024 bodies of s by one wavefront -- each lane
+// a Neumaier sum over its stride, lane partials added in lane order -- and
+// k_heavy_final adds the chunk partials in chunk order.  A system's heavy
+// accelerations therefore depend on that system alone, not on the launch
+// shape or on the other systems of an ensemble.
+__global__ __launch_bounds__(kNT) void k_heavy_partial(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
 {
-    const int k = blockIdx.x / kHeavyChunks, c = blockIdx.x % kHeavyChunks;
-    const uint32_t h = tb.heavy_slot[k];
-    const double x = b.px[h], y = b.py[h], z = b.pz[h];
-    const size_t j0 = n * c / kHeavyChunks, j1 = n * (c + 1) / kHeavyChunks;
-    Neumaier ax, ay, az;
-    for (size_t j = j0 + threadIdx.x; j < j1; j += kNT) {
-        if (j == h) continue;
-        const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
-        const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
-        if (!(r2 > 0.0)) continue;
-        const double invR = 1.0 / sqrt(r2);
-        const double s = p.G * b.m[j] * (invR * invR * invR);
-        ax.add(s * dx);
-        ay.add(s * dy);
-        az.add(s * dz);
-    }
-    __shared__ double part[3][kNT];
-    part[0][threadIdx.x] = ax.value();
-    part[1][threadIdx.x] = ay.value();
-    part[2][threadIdx.x] = az.value();
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        Neumaier t;
-        for (int q = 0; q < kNT; ++q) t.add(part[threadIdx.x][q]);
-        mb.hpart[3 * static_cast<size_t>(blockIdx.x) + threadIdx.x] = t.value();
+    __shared__ double part[kWaves][3][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * kWaves;
+    for (uint32_t item = blockIdx.x * kWaves + w; item < p.heavy_items; item += waves) {
+        uint32_t lo = 0, hi = tb.n_sys;   // the last system whose first item is <= item
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tb.sys[mid].heavy_item <= item) lo = mid;
+            else hi = mid;
+        }
+        const SysScalars* sy = tb.sys + lo;
+        const uint32_t r = item - sy->heavy_item;
+        const uint32_t k = r / sy->heavy_chunks, c = r % sy->heavy_chunks;
+        if (k >= sy->n_heavy) continue;
+        const uint32_t h = sy->heavy_slot[k];
+        const double x = b.px[h], y = b.py[h], z = b.pz[h];
+        const size_t j0 = sy->body_lo + static_cast<size_t>(c) * kHeavyChunk;
+        const size_t j1 = min(j0 + kHeavyChunk, static_cast<size_t>(sy->body_hi));
+        Neumaier ax, ay, az;
+        for (size_t j = j0 + lane; j < j1; j += 64) {
+            if (j == h) continue;
+            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
+            if (!(r2 > 0.0)) continue;
+            const double invR = 1.0 / sqrt(r2);
+            const double s = p.G * b.m[j] * (invR * invR * invR);
+            ax.add(s * dx);
+            ay.add(s * dy);
+            az.add(s * dz);
+        }
+        part[w][0][lane] = ax.value();
+        part[w][1][lane] = ay.value();
+        part[w][2][lane] = az.value();
+        __builtin_amdgcn_wave_barrier();
+        if (lane < 3) {
+            Neumaier t;
+            for (int q = 0; q < 64; ++q) t.add(part[w][lane][q]);
+            mb.hpart[3 * static_cast<size_t>(item) + lane] = t.value();
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-__global__ __launch_bounds__(64) void k_heavy_final(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
+// One thread per (system, heavy rank, axis).
+__global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
 {
-    const int k = blockIdx.x;
-    if (threadIdx.x >= 3) return;
-    const uint32_t h = tb.heavy_slot[k];
-    Neumaier t;
-    for (int c = 0; c < kHeavyChunks; ++c) t.add(mb.hpart[3 * (static_cast<size_t>(k) * kHeavyChunks + c) + threadIdx.x]);
-    const double a = t.value();
-    double* acc = threadIdx.x == 0 ? b.ax : (threadIdx.x == 1 ? b.ay : b.az);
-    double* vel = threadIdx.x == 0 ? b.vx : (threadIdx.x == 1 ? b.vy : b.vz);
-    acc[h] = a;
-    if (p.kick) vel[h] = vel[h] + p.half_dt * a;
+    const uint32_t t = blockIdx.x * kNT + threadIdx.x;
+    const uint32_t a = t % 3, k = (t / 3) % kMaxHeavy, q = t / (3 * kMaxHeavy);
+    if (q >= tb.n_sys) return;
+    const SysScalars* sy = tb.sys + q;
+    if (k >= sy->n_heavy) return;
+    const uint32_t h = sy->heavy_slot[k];
+    Neumaier acc;
+    const size_t first = sy->heavy_item + static_cast<size_t>(k) * sy->heavy_chunks;
+    for (uint32_t c = 0; c < sy->heavy_chunks; ++c) acc.add(mb.hpart[3 * (first + c) + a]);
+    const double v = acc.value();
+    double* accel = a == 0 ? b.ax : (a == 1 ? b.ay : b.az);
+    double* vel = a == 0 ? b.vx : (a == 1 ? b.vy : b.vz);
+    accel[h] = v;
+    if (p.kick) vel[h] = vel[h] + p.half_dt * v;
 }
 
 // ------------------------------------------------------------ schedule
@@ -733,11 +770,12 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 
 }  // namespace
 
-void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, double G, hipStream_t s)
+void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, double G,
+                          hipStream_t s)
 {
-    k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb, mb);
-    k_split_reduce<<<1, 64, 0, s>>>(mb);
-    k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb);
+    k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb);
+    k_sys_extent<<<1, kNT, 0, s>>>(tb, gp);
+    k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
     scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
     k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
     k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, mb.order);
@@ -771,9 +809,10 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
 {
     if (n == 0 || blocks == 0) return;
     k_walk_mixed<<<blocks, kNT, 0, s>>>(b, n, tb, mb, p);
-    if (p.n_heavy > 0) {
-        k_heavy_partial<<<static_cast<unsigned>(p.n_heavy * kHeavyChunks), kNT, 0, s>>>(b, n, tb, mb, p);
-        k_heavy_final<<<static_cast<unsigned>(p.n_heavy), 64, 0, s>>>(b, tb, mb, p);
+    if (p.heavy_items > 0) {
+        const unsigned hb = std::min<unsigned>((p.heavy_items + kWaves - 1) / kWaves, 4096u);
+        k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
+        k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kNT - 1) / kNT, kNT, 0, s>>>(b, tb, mb, p);
     }
 }
 

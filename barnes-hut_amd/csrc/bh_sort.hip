@@ -23,7 +23,9 @@
 // lane are the lanes that share its digit) and a running per-wave digit count
 // in LDS, so that equal digits keep their input order -- the sort is stable,
 // which fixes the order of equal Morton keys -- and the tile's 256 digit
-// counts are chained across tiles by look-back.
+// counts are chained across tiles by look-back.  With several systems a last,
+// stable pass sorts by the system of each payload index, which gives the
+// order (system, key) and keeps every system inside its own index range.
 #include <hip/hip_runtime.h>
 
 #include "bh_internal.hpp"
@@ -36,7 +38,7 @@ constexpr int kIPT = 8;
 constexpr int kTile = kNT * kIPT;  // 2048
 constexpr int kWaves = kNT / 64;
 constexpr int kDigits = 256;
-constexpr int kMaxPasses = 8;
+constexpr int kMaxPasses = 9;   // 8 key digits + the system pass
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 // look-back state at the head of a tmp buffer: counters, then status words
@@ -240,12 +242,13 @@ __host__ __device__ inline SortTmp sort_tmp_at(uint32_t* tmp)
 // Digit counts of every pass: per-block LDS histograms added into the global
 // counts; the last block turns them into each pass's exclusive digit offsets
 // (base) and clears the counts and its counter for the next sort.
-__global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restrict__ key, size_t n, int passes,
-                                                       uint32_t* tmp)
+__global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restrict__ key, size_t n, int key_passes,
+                                                       const uint8_t* __restrict__ sysd, uint32_t* tmp)
 {
     __shared__ uint32_t h[kMaxPasses][kDigits];
     __shared__ bool last;
     const SortTmp st = sort_tmp_at(tmp);
+    const int passes = key_passes + (sysd ? 1 : 0);
     for (int k = threadIdx.x; k < kMaxPasses * kDigits; k += kNT) (&h[0][0])[k] = 0;
     __syncthreads();
     const size_t base = static_cast<size_t>(blockIdx.x) * kTile;
@@ -254,7 +257,8 @@ __global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restric
         const size_t idx = base + static_cast<size_t>(j) * kNT + threadIdx.x;
         if (idx < n) {
             const uint64_t k = key[idx];
-            for (int p = 0; p < passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+            for (int p = 0; p < key_passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+            if (sysd) atomicAdd(&h[key_passes][sysd[idx]], 1u);   // the payload is still the identity
         }
     }
     __syncthreads();
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restric
 
 __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
-                                                   int pass, uint32_t* tmp)
+                                                   int pass, const uint8_t* __restrict__ sysd, uint32_t* tmp)
 {
     __shared__ uint32_t wcnt[kWaves][kDigits];
     __shared__ uint32_t boff[kDigits];
@@ -317,7 +321,8 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
         const bool valid = idx < n;
         kk[j] = valid ? kin[idx] : 0ull;
         vv[j] = valid ? vin[idx] : 0u;
-        const uint32_t d = static_cast<uint32_t>(kk[j] >> shift) & 255u;
+        const uint32_t d = sysd ? (valid ? static_cast<uint32_t>(sysd[vv[j]]) : 0u)
+                                : static_cast<uint32_t>(kk[j] >> shift) & 255u;
         dig[j] = d;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -392,18 +397,19 @@ uint32_t* scan_error_word(uint32_t* scan_tmp) { return scan_tmp + 3; }
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
 
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* tmp, size_t cap, hipStream_t s)
+                      const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s)
 {
     if (n == 0 || n > cap) return;
-    const int passes = (bits + 7) / 8;   // <= 8
+    const int key_passes = (bits + 7) / 8;   // <= 8
+    const int passes = key_passes + (sysd ? 1 : 0);
     const unsigned nt = static_cast<unsigned>(sort_tiles(n));
-    k_sort_hist_all<<<nt, kNT, 0, s>>>(k0, n, passes, tmp);
+    k_sort_hist_all<<<nt, kNT, 0, s>>>(k0, n, key_passes, sysd, tmp);
     uint64_t* ka = k0;
     uint32_t* va = v0;
     uint64_t* kb = k1;
     uint32_t* vb = v1;
     for (int p = 0; p < passes; ++p) {
-        k_sort_pass<<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, tmp);
+        k_sort_pass<<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, p < key_passes ? nullptr : sysd, tmp);
         uint64_t* tk = ka;
         ka = kb;
         kb = tk;

@@ -29,6 +29,12 @@
 // walk index = (#nodes whose range ends after this one) + (#ancestors that end
 // at the same key), and the node after a subtree is #nodes whose range ends at
 // or after the subtree's first key.
+//
+// Systems.  Every system's bodies sit in their own fixed index range and are
+// sorted by (system, key), so leaves come in system order; a boundary between
+// two systems splits nothing (delta = kNoSplit), every binary search runs over
+// the leaves of one system, and each system gets its own root: the forest is
+// exactly the union of the trees the systems would build alone.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -73,15 +79,36 @@ __device__ inline int split_depth(uint64_t a, uint64_t b)
 __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
 
+__device__ inline uint32_t leaf_sys(const TreeBuffers& tb, uint32_t j) { return tb.bsys[tb.leaf_first[j]]; }
+
+// leaves [lo, hi) of system s
+__device__ inline void sys_leaves(const TreeBuffers& tb, uint32_t M, uint32_t s, uint32_t& lo, uint32_t& hi)
+{
+    lo = tb.sys[s].leaf_lo;
+    hi = (s + 1 < tb.n_sys) ? tb.sys[s + 1].leaf_lo : M;
+}
+
+// first index in u[lo..hi) with u[idx] >= x
+__device__ inline uint32_t lower_bound_in(const uint64_t* __restrict__ u, uint32_t lo, uint32_t hi, uint64_t x)
+{
+    return lo + lower_bound_u64(u + lo, hi - lo, x);
+}
+
 // ------------------------------------------------------ kick/drift/box
-__global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int kick, double half_dt,
-                                                        int drift, double dt, StepScalars* sc,
+// Block (s, k) of the grid (per_sys blocks per system) strides over system
+// s's bodies; the block that finishes last reduces every system's partial
+// boxes, one wavefront per system.  Min and max are exact, so the box does
+// not depend on the partition.
+__global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, double half_dt, int drift, double dt,
+                                                        TreeBuffers tb, uint32_t per_sys,
                                                         double* __restrict__ partials)
 {
+    const uint32_t sys = blockIdx.x / per_sys, part = blockIdx.x % per_sys;
+    const size_t i0 = tb.sys[sys].body_lo, i1 = tb.sys[sys].body_hi;
     double lo[3] = {INFINITY, INFINITY, INFINITY};
     double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const size_t stride = static_cast<size_t>(gridDim.x) * kNT;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x; i < n; i += stride) {
+    const size_t stride = static_cast<size_t>(per_sys) * kNT;
+    for (size_t i = i0 + static_cast<size_t>(part) * kNT + threadIdx.x; i < i1; i += stride) {
         double v[3] = {b.vx[i], b.vy[i], b.vz[i]};
         if (kick) {
             // src/main.cpp:196-198: v += 0.5 * dt * a
@@ -125,6 +152,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int 
             red[3 + a][w] = hi[a];
         }
     __syncthreads();
+    StepScalars* sc = tb.sc;
     if (threadIdx.x == 0) {
         for (int a = 0; a < 3; ++a) {
             double l = red[a][0], h = red[3 + a][0];
@@ -142,55 +170,46 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, size_t n, int 
     __syncthreads();
     if (!last) return;
     __threadfence();
-    // the last block reduces every block's partial box in parallel
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = INFINITY;
-        hi[a] = -INFINITY;
-    }
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += kNT) {
+    // the last block: one wavefront per system reduces its per_sys partials
+    for (uint32_t q = static_cast<uint32_t>(w); q < tb.n_sys; q += kNT / 64) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            lo[a] = dmin(lo[a], __hip_atomic_load(&partials[6 * k + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            hi[a] = dmax(hi[a], __hip_atomic_load(&partials[6 * k + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            lo[a] = INFINITY;
+            hi[a] = -INFINITY;
         }
-    }
+        for (uint32_t k = static_cast<uint32_t>(lane); k < per_sys; k += 64) {
+            const size_t blk = static_cast<size_t>(q) * per_sys + k;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        for (int o = 32; o > 0; o >>= 1) {
-            lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
-            hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
-        }
-    }
-    if (lane == 0)
-        for (int a = 0; a < 3; ++a) {
-            red[a][w] = lo[a];
-            red[3 + a][w] = hi[a];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double bb[6];
-        for (int a = 0; a < 3; ++a) {
-            double l = red[a][0], h = red[3 + a][0];
-            for (int k = 1; k < kNT / 64; ++k) {
-                l = dmin(l, red[a][k]);
-                h = dmax(h, red[3 + a][k]);
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = dmin(lo[a], __hip_atomic_load(&partials[6 * blk + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                hi[a] = dmax(hi[a], __hip_atomic_load(&partials[6 * blk + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             }
-            bb[a] = l;
-            bb[3 + a] = h;
         }
-        for (int k = 0; k < 6; ++k) sc->bb[k] = bb[k];
-        // src/morton_keys.cpp:71-73
-        for (int a = 0; a < 3; ++a) sc->span[a] = (bb[3 + a] == bb[a]) ? 1.0 : (bb[3 + a] - bb[a]);
-        // src/traversal.cpp:31-43
-        const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
-        double L = ex;
-        if (L < ey) L = ey;
-        if (L < ez) L = ez;
-        for (int d = 0; d <= kMaxDepth; ++d) {
-            const double side = L / static_cast<double>(1ull << d);
-            sc->side2[d] = side * side;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
+                hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
+            }
         }
+        if (lane == 0) {
+            SysScalars* sy = tb.sys + q;
+            double bb[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+            for (int k = 0; k < 6; ++k) sy->bb[k] = bb[k];
+            // src/morton_keys.cpp:71-73
+            for (int a = 0; a < 3; ++a) sy->span[a] = (bb[3 + a] == bb[a]) ? 1.0 : (bb[3 + a] - bb[a]);
+            // src/traversal.cpp:31-43
+            const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
+            double L = ex;
+            if (L < ey) L = ey;
+            if (L < ez) L = ez;
+            for (int d = 0; d <= kMaxDepth; ++d) {
+                const double side = L / static_cast<double>(1ull << d);
+                sy->side2[d] = side * side;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
         sc->error = 0;
         sc->group_counter = 0;
         sc->block_counter = 0;
@@ -218,30 +237,29 @@ __device__ inline uint32_t quantise(double v, double lo, double span)
     return static_cast<uint32_t>(t * 2097152.0);
 }
 
-__global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, const StepScalars* __restrict__ sc,
-                                              uint64_t* __restrict__ key, uint32_t* __restrict__ perm)
+__global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, TreeBuffers tb)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (i >= n) return;
-    const uint32_t qx = quantise(b.px[i], sc->bb[0], sc->span[0]);
-    const uint32_t qy = quantise(b.py[i], sc->bb[1], sc->span[1]);
-    const uint32_t qz = quantise(b.pz[i], sc->bb[2], sc->span[2]);
-    key[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
-    perm[i] = static_cast<uint32_t>(i);
+    const SysScalars* sy = tb.sys + tb.bsys[i];
+    const uint32_t qx = quantise(b.px[i], sy->bb[0], sy->span[0]);
+    const uint32_t qy = quantise(b.py[i], sy->bb[1], sy->span[1]);
+    const uint32_t qz = quantise(b.pz[i], sy->bb[2], sy->span[2]);
+    tb.key[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+    tb.perm[i] = static_cast<uint32_t>(i);
 }
 
 // ---------------------------------------------------------- permute
-__global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, const uint32_t* __restrict__ perm,
-                                                 const uint64_t* __restrict__ key, uint32_t* __restrict__ flag,
-                                                 uint32_t* __restrict__ heavy_slot, size_t n)
+__global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, TreeBuffers tb, size_t n)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (i >= n) return;
-    const uint32_t j = perm[i];
+    const uint32_t j = tb.perm[i];
+    const uint8_t sys = tb.bsys[i];   // = bsys[j]: sorting stays inside a system
     const int8_t hr = src.hrank[j];
     dst.hrank[i] = hr;
     dst.pcost[i] = src.pcost[j];
-    if (hr >= 0) heavy_slot[hr] = static_cast<uint32_t>(i);
+    if (hr >= 0) tb.sys[sys].heavy_slot[hr] = static_cast<uint32_t>(i);
     dst.px[i] = src.px[j];
     dst.py[i] = src.py[j];
     dst.pz[i] = src.pz[j];
@@ -250,7 +268,7 @@ __global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, const u
     dst.vz[i] = src.vz[j];
     dst.m[i] = src.m[j];
     dst.id[i] = src.id[j];
-    flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+    tb.flag[i] = (i == 0 || tb.key[i] != tb.key[i - 1] || tb.bsys[i - 1] != sys) ? 1u : 0u;
 }
 
 // ------------------------------------------------------------- tree
@@ -267,6 +285,8 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
     if (f) {
         tb.ukey[j] = key[i];
         tb.leaf_first[j] = static_cast<uint32_t>(i);
+        SysScalars* sy = tb.sys + tb.bsys[i];
+        if (i == sy->body_lo) sy->leaf_lo = j;
     }
     if (i == 0) tb.leaf_first[tb.sc->n_leaves] = static_cast<uint32_t>(n);
 }
@@ -325,14 +345,21 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
     }
     uint32_t isrep = 0;
     if (M > 1 && j < M - 1) {
-        const uint64_t a = tb.ukey[j];
-        const int d = split_depth(a, tb.ukey[j + 1]);
-        tb.delta[j] = static_cast<uint8_t>(d);
-        const uint32_t lo = lower_bound_u64(tb.ukey, M, prefix_at(a, d));
-        const uint64_t first_child = prefix_at(tb.ukey[lo], d + 1);
-        const uint32_t rep = lower_bound_u64(tb.ukey, M, first_child + stride_at(d + 1)) - 1;
-        tb.rep[j] = rep;
-        isrep = (rep == j) ? 1u : 0u;
+        const uint32_t sys = leaf_sys(tb, static_cast<uint32_t>(j));
+        if (leaf_sys(tb, static_cast<uint32_t>(j) + 1) != sys) {
+            tb.delta[j] = kNoSplit;   // the last leaf of a system
+        } else {
+            uint32_t L0, L1;
+            sys_leaves(tb, M, sys, L0, L1);
+            const uint64_t a = tb.ukey[j];
+            const int d = split_depth(a, tb.ukey[j + 1]);
+            tb.delta[j] = static_cast<uint8_t>(d);
+            const uint32_t lo = lower_bound_in(tb.ukey, L0, L1, prefix_at(a, d));
+            const uint64_t first_child = prefix_at(tb.ukey[lo], d + 1);
+            const uint32_t rep = lower_bound_in(tb.ukey, L0, L1, first_child + stride_at(d + 1)) - 1;
+            tb.rep[j] = rep;
+            isrep = (rep == j) ? 1u : 0u;
+        }
     }
     tb.is_rep[j] = isrep;
 }
@@ -343,17 +370,19 @@ __global__ __launch_bounds__(kNT) void k_branch_init(size_t n, TreeBuffers tb)
     const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (j >= n) return;
     const uint32_t M = tb.sc->n_leaves;
-    if (!(M > 1 && j < M - 1)) return;
+    if (!(M > 1 && j < M - 1) || tb.delta[j] == kNoSplit) return;
     const uint32_t rep = tb.rep[j];
     const bool is_rep = (rep == j);
     tb.rep[j] = tb.bidx[rep];   // from here on: branching index of j's node
     if (!is_rep) return;
+    uint32_t L0, L1;
+    sys_leaves(tb, M, leaf_sys(tb, static_cast<uint32_t>(j)), L0, L1);
     const uint32_t bi = tb.bidx[j];
     const int d = tb.delta[j];
     const uint64_t p = prefix_at(tb.ukey[j], d);
     tb.bdepth[bi] = static_cast<uint8_t>(d);
-    tb.blo[bi] = lower_bound_u64(tb.ukey, M, p);
-    tb.bhi[bi] = lower_bound_u64(tb.ukey, M, p + stride_at(d)) - 1;
+    tb.blo[bi] = lower_bound_in(tb.ukey, L0, L1, p);
+    tb.bhi[bi] = lower_bound_in(tb.ukey, L0, L1, p + stride_at(d)) - 1;
     int32_t* c = tb.child + 8 * static_cast<size_t>(bi);
 #pragma unroll
     for (int k = 0; k < 8; ++k) c[k] = -1;
@@ -373,11 +402,11 @@ __global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
         hi = tb.bhi[t - M];
         atomicAdd(&tb.cnt_hi[hi], 1u);
     }
-    const int dl = lo > 0 ? tb.delta[lo - 1] : -1;
-    const int dr = hi + 1 < M ? tb.delta[hi] : -1;
+    const int dl = (lo > 0 && tb.delta[lo - 1] != kNoSplit) ? tb.delta[lo - 1] : -1;
+    const int dr = (hi + 1 < M && tb.delta[hi] != kNoSplit) ? tb.delta[hi] : -1;
     if (dl < 0 && dr < 0) {
         tb.parent[t] = kNone;
-        tb.sc->root = static_cast<uint32_t>(t);
+        tb.sys[leaf_sys(tb, lo)].root = static_cast<uint32_t>(t);
         return;
     }
     const uint32_t bnd = (dl > dr) ? lo - 1 : hi;
@@ -516,7 +545,7 @@ __global__ __launch_bounds__(kNT) void k_walk_records(size_t n, TreeBuffers tb)
         nd.z = finish_com(r[3], m);
         nd.m = 0.0 + m;
     }
-    nd.side2 = tb.sc->side2[leaf ? kMaxDepth - 1 : test_depth];
+    nd.side2 = tb.sys[0].side2[leaf ? kMaxDepth - 1 : test_depth];   // the plain form runs one system
     nd.skip = skip;
     uint32_t meta = test_depth | (top << kMetaTopShift);
     if (leaf) {
@@ -536,23 +565,26 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 }  // namespace
 
 void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
-                           StepScalars* sc, double* partials, hipStream_t s)
+                           const TreeBuffers& tb, double* partials, hipStream_t s)
 {
+    // blocks per system: up to 512 blocks in all, at least one per system
     unsigned g = nblk(n);
-    if (g > 512) g = 512;   // partials hold 1024 blocks
-    k_kick_drift_box<<<g, kNT, 0, s>>>(b, n, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, sc, partials);
+    if (g > 512) g = 512;
+    uint32_t per_sys = g / tb.n_sys;
+    if (per_sys < 1) per_sys = 1;
+    static_assert(static_cast<unsigned>(kMaxSystems) <= kBoxBlocks, "one partial box per block");
+    k_kick_drift_box<<<per_sys * tb.n_sys, kNT, 0, s>>>(b, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, tb, per_sys,
+                                                        partials);
 }
 
-void launch_keys(const Bodies& b, size_t n, const StepScalars* sc, uint64_t* key, uint32_t* perm,
-                 hipStream_t s)
+void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s)
 {
-    k_keys<<<nblk(n), kNT, 0, s>>>(b, n, sc, key, perm);
+    k_keys<<<nblk(n), kNT, 0, s>>>(b, n, tb);
 }
 
-void launch_permute(const Bodies& src, const Bodies& dst, const uint32_t* perm, const uint64_t* key,
-                    uint32_t* flag, uint32_t* heavy_slot, size_t n, hipStream_t s)
+void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s)
 {
-    k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, perm, key, flag, heavy_slot, n);
+    k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, tb, n);
 }
 
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s)

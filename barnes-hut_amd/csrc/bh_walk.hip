@@ -31,17 +31,18 @@ constexpr int kNT = 256;
 
 // Full-chain bookkeeping for counters: the first depth in [top, bottom] at
 // which the reference's test passes, given theta^2 r^2.
-__device__ inline uint32_t first_pass_depth(const StepScalars* sc, uint32_t top, uint32_t bottom, double t2r2)
+__device__ inline uint32_t first_pass_depth(const SysScalars* sy, uint32_t top, uint32_t bottom, double t2r2)
 {
     uint32_t e = top;
-    while (e < bottom && !(sc->side2[e] < t2r2)) ++e;
+    while (e < bottom && !(sy->side2[e] < t2r2)) ++e;
     return e;
 }
 
 template <bool kCount>
 __global__ __launch_bounds__(kNT) void k_walk_plain(Bodies b, size_t n, const WalkNode* __restrict__ walk,
                                                     const uint32_t* __restrict__ self_w,
-                                                    const StepScalars* __restrict__ sc, WalkParams p,
+                                                    const StepScalars* __restrict__ sc,
+                                                    const SysScalars* __restrict__ sy, WalkParams p,
                                                     int64_t* __restrict__ cnt_accept, int64_t* __restrict__ cnt_open,
                                                     int64_t* __restrict__ cnt_test)
 {
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kNT) void k_walk_plain(Bodies b, size_t n, const Wa
                         ++na;
                         // the reference opens the chain cells above the first passing depth
                         const uint32_t last = (nd.meta & kMetaLeaf) ? (is_self ? kMaxDepth - 1 : kMaxDepth) : bottom;
-                        no += first_pass_depth(sc, top, last, t2r2) - top;
+                        no += first_pass_depth(sy, top, last, t2r2) - top;
                     } else if (is_self) {
                         no += kMaxDepth - top;
                     } else {
@@ -138,9 +139,10 @@ void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const W
     const unsigned g = static_cast<unsigned>((n + kNT - 1) / kNT);
     if (g == 0) return;
     if (cnt_accept && cnt_open && cnt_test)
-        k_walk_plain<true><<<g, kNT, 0, s>>>(b, n, tb.walk, tb.self_w, tb.sc, p, cnt_accept, cnt_open, cnt_test);
+        k_walk_plain<true><<<g, kNT, 0, s>>>(b, n, tb.walk, tb.self_w, tb.sc, tb.sys, p, cnt_accept, cnt_open,
+                                             cnt_test);
     else
-        k_walk_plain<false><<<g, kNT, 0, s>>>(b, n, tb.walk, tb.self_w, tb.sc, p, nullptr, nullptr, nullptr);
+        k_walk_plain<false><<<g, kNT, 0, s>>>(b, n, tb.walk, tb.self_w, tb.sc, tb.sys, p, nullptr, nullptr, nullptr);
 }
 
 }  // namespace bh

@@ -1,12 +1,14 @@
 #!/usr/bin/env python3
 """Body-steps/s of the MI355X Barnes-Hut engine on one GPU.
 
-Workload: 300,149 bodies from the synthetic heliocentric generator (central
-mass, 8 planets, main-belt asteroids; seed 0), theta = 1.05, dt = 1 h, the
-drift-first KDK step of the reference's `--fc tree` path.  A step is one pass
-of the whole path over the resident bodies: half kick, drift, box, Morton
-keys, radix sort + gather, octree build, walk, half kick.  Inputs are in HBM
-before the timed region starts.
+Default workload (--config c1): 300,149 bodies from the synthetic
+heliocentric generator (central mass, 8 planets, main-belt asteroids; seed 0),
+theta = 1.05, dt = 1 h, the drift-first KDK step of the reference's
+`--fc tree` path.  A step is one pass of the whole path over the resident
+bodies: half kick, drift, box, Morton keys, radix sort + gather, octree
+build, walk, half kick.  Inputs are in HBM before the timed region starts;
+the timed steps are the library's bh_step, which replays the step as a HIP
+graph.
 
 Prints one JSON line.  `value` is body-steps/s of the form given by --form
 (default: LEARNED, the mixed-precision walk with the learned schedule, whose
@@ -15,9 +17,16 @@ plain FP64 walk (bit-identical to the reference) and the mixed walk without
 the cost model reported beside it; the maximum relative force error of each
 form against an FP64 direct sum on 4,096 seeded sample bodies plus the Sun
 and planets; the roofline of the dominant walk kernel (algorithmic flops per
-launch / measured walk time); and a CPU baseline: the reference's own code
+launch / its mean duration, HIP events on the library stream over eager steps
+that follow the timed region); and a CPU baseline: the reference's own code
 (oracle/_ref, Release-like flags, OpenMP) on a bounded sample of the same
 workload.
+
+Other configurations (--config): c0 (19,054 bodies, plain), c1-theta0.5,
+c2 (4,194,304 bodies with a Kuiper belt, theta 0.7), c3 (1,048,576 bodies
+with Trojan clouds, learned schedule), c4 (64 systems of log-uniform size in
+[1e4, 2e5], one batched step, autotuned group shape, error per system on 512
+samples each).
 """
 from __future__ import annotations
 
@@ -52,21 +61,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50):
+PHASE_STEPS = 5
+
+
+def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, systems=None, tune=False):
     """Fresh engine; the LEARNED form first trains its cost model on the
     first `train_steps` steps (untimed), then every form runs `warmup`
-    untimed and `steps` timed steps."""
+    untimed and `steps` timed steps (graph replay).  PHASE_STEPS eager steps
+    after the timed region give the per-phase device times."""
     eng = bh.Engine(form=form, theta=theta)
     eng.set_bodies(pos, vel, mass)
+    if systems is not None:
+        eng.set_systems(systems)
+    tuned = eng.autotune(reps=2) if tune else None
     if form == bh.FORM_LEARNED:
         eng.model_configure(train_steps=train_steps)
         eng.step(train_steps, HOUR)
     eng.step(warmup, HOUR)
+    r0 = eng.graph_replays()
     t0 = time.perf_counter()
     eng.step(steps, HOUR)          # synchronous on return
     t1 = time.perf_counter()
+    replays = eng.graph_replays() - r0
+    eng.set_graph(False)
+    e0 = time.perf_counter()
+    eng.step(PHASE_STEPS, HOUR)
+    e1 = time.perf_counter()
+    eng.set_graph(True)
     phases, nph = eng.step_phase_ms()
-    return eng, (t1 - t0), phases
+    return eng, (t1 - t0), phases, dict(graph_steps=replays, eager_ms_per_step=(e1 - e0) / PHASE_STEPS * 1e3,
+                                        autotune=tuned)
 
 
 def force_error(bh, eng, n, samples, seed=0):
@@ -75,13 +99,36 @@ def force_error(bh, eng, n, samples, seed=0):
     where = np.empty(n, np.int64)
     where[ids] = np.arange(n)
     rng = np.random.default_rng(seed)
-    sample_ids = np.concatenate([np.arange(9), rng.choice(np.arange(9, n), samples, replace=False)])
+    sample_ids = np.concatenate([np.arange(9), rng.choice(np.arange(9, n), min(samples, n - 9), replace=False)])
     si = where[sample_ids]
     ds = eng.direct_sum(si)
     err = np.linalg.norm(st["acc"][si] - ds, axis=1) / np.linalg.norm(ds, axis=1)
     return dict(max_all=float(err.max()), max_asteroids=float(err[9:].max()),
                 max_central_and_planets=float(err[:9].max()), median=float(np.median(err)),
-                p99=float(np.quantile(err, 0.99)), samples=int(samples), includes_ids_0_8=True)
+                p99=float(np.quantile(err, 0.99)), samples=int(len(sample_ids) - 9), includes_ids_0_8=True)
+
+
+def ensemble_error(bh, eng, sizes, samples, seed=0):
+    """Per system: ids 0-8 (central mass, planets) plus `samples` seeded
+    asteroids, against the device FP64 direct sum over that system."""
+    st = eng.get_bodies()
+    off = eng.systems()
+    rng = np.random.default_rng(seed)
+    per = []
+    for k, ns in enumerate(sizes):
+        lo = int(off[k])
+        ids = st["ids"][lo:lo + ns].astype(np.int64) - lo
+        where = np.empty(ns, np.int64)
+        where[ids] = np.arange(ns)
+        pick = np.concatenate([np.arange(9), 9 + rng.choice(ns - 9, min(samples, ns - 9), replace=False)])
+        si = lo + where[pick]
+        ds = eng.direct_sum(si)
+        err = np.linalg.norm(st["acc"][si] - ds, axis=1) / np.linalg.norm(ds, axis=1)
+        per.append(dict(n=int(ns), max_all=float(err.max()), max_asteroids=float(err[9:].max()),
+                        median=float(np.median(err))))
+    return dict(max_all=max(p["max_all"] for p in per), max_asteroids=max(p["max_asteroids"] for p in per),
+                median_of_medians=float(np.median([p["median"] for p in per])), samples_per_system=samples,
+                includes_ids_0_8=True, per_system=per)
 
 
 def plain_walk_work(bh, eng, theta):
@@ -136,18 +183,50 @@ def cpu_baseline(pos, vel, mass, theta, steps):
                       f"{what}, OpenMP threads {cores}, {dt:.1f} s"}
 
 
+CONFIGS = {
+    "c1": dict(kind="belt", n=300149, theta=1.05, steps=100, form="learned", forms="plain,mixed,learned",
+               workload="heliocentric belt system, 300149 bodies (central mass, 8 planets, asteroids), theta 1.05, "
+                        "dt 1 h, drift-first KDK"),
+    "c1-theta0.5": dict(kind="belt", n=300149, theta=0.5, steps=100, form="learned", forms="plain,mixed,learned",
+                        workload="heliocentric belt system, 300149 bodies, theta 0.5, dt 1 h, drift-first KDK"),
+    "c0": dict(kind="belt", n=19054, theta=1.05, steps=200, form="plain", forms="plain",
+               workload="heliocentric belt system, 19054 bodies, plain FP64 walk, theta 1.05, dt 1 h, 200 KDK steps"),
+    "c2": dict(kind="kuiper", n=4194304, theta=0.7, steps=48, form="learned", forms="mixed,learned",
+               workload="belt + 30% Kuiper belt, 4194304 bodies, theta 0.7, dt 1 h, precision form at scale"),
+    "c3": dict(kind="trojan", n=1048576, theta=1.05, steps=200, form="learned", forms="plain,mixed,learned",
+               workload="belt 60% + Jupiter Trojans 2 x 15% + Hildas 10%, 1048576 bodies, theta 1.05, learned "
+                        "schedule"),
+    "c4": dict(kind="belt", ensemble=64, nmin=10000, nmax=200000, theta=1.05, steps=100, form="learned",
+               forms="mixed,learned",
+               workload="64 independent belt systems (seeds 0..63), N log-uniform in [1e4, 2e5], one batched "
+                        "graph-replayed step, autotuned group shape, theta 1.05, dt 1 h"),
+}
+
+
+def traffic_record(kernel, config):
+    """Calibrated HBM bytes per launch of `kernel` on `config`, from a
+    committed rocprofv3 summary (profiles/<kernel>.<config>.traffic.json), or
+    None."""
+    f = ROOT / "profiles" / f"{kernel}.{config}.traffic.json"
+    if not f.exists():
+        return None
+    rec = json.loads(f.read_text())
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="c1")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the config's)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=300149)
-    ap.add_argument("--theta", type=float, default=1.05)
-    ap.add_argument("--form", choices=["plain", "mixed", "learned"], default="learned")
+    ap.add_argument("--n", type=int, default=None, help="override the body count")
+    ap.add_argument("--theta", type=float, default=None)
+    ap.add_argument("--form", choices=["plain", "mixed", "learned"], default=None)
     ap.add_argument("--samples", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain,mixed,learned",
+    ap.add_argument("--forms", default=None,
                     help="forms to run besides --form (comma list); the plain form gives speedup_vs_plain")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (calibrates rocprofv3 FETCH_SIZE / "
@@ -169,25 +248,51 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 10}))
         return
 
-    bh = load_bh()
-    forms = {"plain": bh.FORM_PLAIN, "mixed": bh.FORM_MIXED, "learned": bh.FORM_LEARNED}
-    log(f"bench: generating {args.n} bodies")
-    pos, vel, mass = bh.generate(bh.GEN_BELT, 0, args.n)
-    steps, warmup = args.steps, args.warmup
+    cfg = CONFIGS[args.config]
+    theta = cfg["theta"] if args.theta is None else args.theta
+    steps = cfg["steps"] if args.steps is None else args.steps
+    warmup = args.warmup
+    form = cfg["form"] if args.form is None else args.form
+    forms_arg = cfg["forms"] if args.forms is None else args.forms
 
+    bh = load_bh()
+    kinds = {"belt": bh.GEN_BELT, "kuiper": bh.GEN_KUIPER, "trojan": bh.GEN_TROJAN}
+    systems = None
+    if "ensemble" in cfg:
+        sizes = [int(x) for x in bh.ensemble_sizes(0, cfg["ensemble"], cfg["nmin"], cfg["nmax"])]
+        log(f"bench: generating {len(sizes)} systems, {sum(sizes)} bodies")
+        parts = [bh.generate(kinds[cfg["kind"]], s, ns) for s, ns in enumerate(sizes)]
+        pos = np.concatenate([q[0] for q in parts])
+        vel = np.concatenate([q[1] for q in parts])
+        mass = np.concatenate([q[2] for q in parts])
+        systems = sizes
+        n = int(sum(sizes))
+    else:
+        n = cfg["n"] if args.n is None else args.n
+        log(f"bench: generating {n} bodies")
+        pos, vel, mass = bh.generate(kinds[cfg["kind"]], 0, n)
+
+    forms = {"plain": bh.FORM_PLAIN, "mixed": bh.FORM_MIXED, "learned": bh.FORM_LEARNED}
     results = {}
-    wanted = set(args.forms.split(",")) | {args.form}
+    wanted = set(f for f in forms_arg.split(",") if f) | {form}
+    if systems is not None:
+        wanted.discard("plain")
     for name in ("plain", "mixed", "learned"):
         if name not in wanted:
             continue
         log(f"bench: {name}, warmup {warmup}, steps {steps}")
-        eng, wall, ph = run_form(bh, forms[name], pos, vel, mass, args.theta, steps, warmup)
-        r = dict(value=args.n * steps / wall, ms_per_step=wall / steps * 1e3,
+        eng, wall, ph, extra = run_form(bh, forms[name], pos, vel, mass, theta, steps, warmup, systems=systems,
+                                        tune=systems is not None and name != "plain")
+        r = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3,
                  phase_ms={"kick_drift_box_keys": ph[0], "sort_gather": ph[1], "tree_groups_schedule": ph[2],
                            "walk_kick": ph[3], "total_device": ph[4]},
-                 force_error=force_error(bh, eng, args.n, args.samples))
+                 phase_note=f"device phases from {PHASE_STEPS} eager steps after the timed region", **extra)
+        if systems is not None:
+            r["force_error"] = ensemble_error(bh, eng, systems, 512)
+        else:
+            r["force_error"] = force_error(bh, eng, n, args.samples)
         if name == "plain":
-            flops, ma, mt, mo = plain_walk_work(bh, eng, args.theta)
+            flops, ma, mt, mo = plain_walk_work(bh, eng, theta)
             r["walk_work"] = dict(flops_per_launch=flops, accepted_per_body=ma, tested_per_body=mt,
                                   opened_full_chain_per_body=mo)
             r["walk_tflops"] = flops / (ph[3] * 1e-3) / 1e12
@@ -200,43 +305,50 @@ def main():
             r["model"] = dict(ready=ready, samples=samples, updates=updates, train_loss=loss,
                               trained_steps=50, note="trained on the first 50 steps before the timed region")
         results[name] = r
-        log(f"bench: {name} {r['value']:.4g} body-steps/s, {r['ms_per_step']:.3f} ms/step, "
-            f"phases {np.round(ph, 3).tolist()}, max err {r['force_error']['max_all']:.3g}")
+        log(f"bench: {name} {r['value']:.4g} body-steps/s, {r['ms_per_step']:.3f} ms/step "
+            f"(eager {extra['eager_ms_per_step']:.3f}), phases {np.round(ph, 3).tolist()}, "
+            f"max err {r['force_error']['max_all']:.3g}")
         eng.close()
 
-    form = args.form
     head = results[form]
-    plain = results.get("plain", head)
+    plain = results.get("plain")
     if form == "plain":
-        kern, peak, bound_note = "k_walk_plain (FP64 walk + fused half kick)", FP64_PEAK_TFLOPS, "FP64 vector"
+        kname, kern, peak, bound_note = "k_walk_plain", "k_walk_plain (FP64 walk + fused half kick)", \
+            FP64_PEAK_TFLOPS, "FP64 vector"
     else:
-        kern, peak, bound_note = "k_walk_mixed (FP32 group walk + FP64 heavy term + fused half kick)", \
-            FP32_PEAK_TFLOPS, "FP32 vector (published peak counts packed FP32, which this build does not use: " \
-                              "the non-packed ceiling is 78.6)"
+        kname, kern, peak, bound_note = "k_walk_mixed", \
+            "k_walk_mixed (FP32 group walk + FP64 heavy term + fused half kick)", FP32_PEAK_TFLOPS, \
+            "FP32 vector (published peak counts packed FP32, which this build does not use: the non-packed " \
+            "ceiling is 78.6)"
+    tr = traffic_record(kname, args.config)
     roofline = {"bound": "mfma", "achieved": head["walk_tflops"], "peak": peak, "unit": "TFLOP/s",
-                "frac": head["walk_tflops"] / peak, "traffic": None, "kernel": kern,
+                "frac": head["walk_tflops"] / peak, "traffic": tr["bytes_per_launch"] if tr else None,
+                "kernel": kern,
                 "note": f"compute roofline at the {bound_note} peak; the walk runs on VALU (a gather, no dense "
                         "contraction, so no MFMA).  Algorithmic flops per launch are counted from the walk's own "
-                        "list lengths / node tests; time is the kernel's mean over the timed steps (HIP events on "
-                        "the library stream)",
+                        "list lengths / node tests; time is the walk phase's mean over eager steps right after "
+                        "the timed region (HIP events on the library stream; the phase is the walk kernel plus "
+                        "the small heavy-body kernels)",
                 "flops_per_launch": head["walk_work"]["flops_per_launch"], "walk_ms": head["phase_ms"]["walk_kick"]}
+    if tr:
+        roofline["traffic_source"] = tr.get("source")
+        roofline["traffic_gbs"] = tr["bytes_per_launch"] / (head["phase_ms"]["walk_kick"] * 1e-3) / 1e9
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and systems is None:
         log(f"bench: CPU baseline, {args.cpu_steps} steps")
-        cpu = cpu_baseline(pos, vel, mass, args.theta, args.cpu_steps)
+        cpu = cpu_baseline(pos, vel, mass, theta, args.cpu_steps)
 
     line = {
         "metric": METRIC, "value": head["value"], "unit": "body-steps/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if form == "plain" else "f32+f64",
-        "data": "synthetic (project generator, seed 0)",
-        "config": {"workload": f"heliocentric belt system, {args.n} bodies (central mass, 8 planets, asteroids), "
-                               f"theta {args.theta}, dt 1 h, drift-first KDK",
-                   "n_bodies": args.n, "theta": args.theta, "dt_days": HOUR, "form": form},
+        "data": "synthetic (project generator" + (", seeds 0..63)" if systems else ", seed 0)"),
+        "config": {"workload": cfg["workload"], "name": args.config, "n_bodies": n, "theta": theta,
+                   "dt_days": HOUR, "form": form, **({"systems": len(systems)} if systems else {})},
         "max_rel_force_err": head["force_error"]["max_all"],
-        "speedup_vs_plain": head["value"] / plain["value"],
+        "speedup_vs_plain": head["value"] / plain["value"] if plain else None,
         "forms": results,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -106,8 +106,16 @@ int bh_compute_accelerations(bh_ctx *ctx, int form, double theta, double G, doub
 
 /* nsteps iterations of the drift-first KDK loop (src/main.cpp:191-281):
  * half kick (skipped while t == 0), drift, force evaluation with ctx's form,
- * theta, G and soft2, half kick; t += dt.  Everything stays on the device. */
+ * theta, G and soft2, half kick; t += dt.  Everything stays on the device.
+ * Steady steps (t > 0, and for LEARNED once the model is trained) are
+ * captured into a HIP graph per parity of the body double buffer and
+ * replayed; bh_set_graph(ctx, 0) runs every step eagerly instead.  Graph
+ * steps record no phase events (bh_get_step_phase_ms covers eager steps). */
 int bh_step(bh_ctx *ctx, int64_t nsteps, double dt);
+/* Graph replay of steady steps on (default) or off; *replays (may be NULL)
+ * counts the graph steps run so far. */
+int bh_set_graph(bh_ctx *ctx, int on);
+int bh_get_graph_replays(bh_ctx *ctx, int64_t *replays);
 /* Simulation time t accumulated by bh_step. */
 double bh_time(const bh_ctx *ctx);
 /* Number of iterations `for (t = 0; t < t_end; t += dt)` performs
@@ -207,10 +215,52 @@ int bh_get_groups(bh_ctx *ctx, int64_t *n_groups, int64_t capacity, uint32_t *fi
 int bh_get_group_timeline(bh_ctx *ctx, int64_t capacity, uint64_t *t_start, uint64_t *t_end, uint32_t *worker);
 
 /* FP64 direct sum on the device for m sample bodies (indices into the
- * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 with
- * Neumaier-compensated sums, r^2 = |d|^2 + soft2.  The reference for force
- * errors; acc_out is m x 3. */
+ * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 over the
+ * bodies j of i's system, with Neumaier-compensated sums, r^2 = |d|^2 +
+ * soft2.  The reference for force errors; acc_out is m x 3. */
 int bh_direct_sum(bh_ctx *ctx, int64_t m, const int64_t *idx, double G, double soft2, bh_vec3 *acc_out);
+
+/* ------------------------------------------------ ensembles -----------
+ * No counterpart in the reference (one process simulates one system): the
+ * batched ensemble form steps many independent systems in one launch
+ * sequence.  After bh_set_bodies with the systems' bodies concatenated,
+ * bh_set_systems(ctx, S, counts) splits them into S <= BH_MAX_SYSTEMS
+ * systems of counts[s] >= 1 bodies (sum = n); system s keeps the index range
+ * [off[s], off[s+1]) for good (sorting permutes inside a system only).
+ * Every system gets its own box, keys, tree, walk groups and heavy set, so
+ * its forces are bit-identical to a run of that system alone.  The MIXED and
+ * LEARNED forms run ensembles; PLAIN returns BH_ERR_UNSUPPORTED for S > 1.
+ * bh_set_bodies resets to one system. */
+#define BH_MAX_SYSTEMS 256
+int bh_set_systems(bh_ctx *ctx, int64_t n_systems, const int64_t *counts);
+/* *n_systems gets S; offsets (S + 1 entries, may be NULL) the body ranges. */
+int bh_get_systems(bh_ctx *ctx, int64_t *n_systems, int64_t capacity, int64_t *offsets);
+
+/* Walk-group shape of the mixed forms: at most gmax (1..64) bodies per group
+ * and group boxes no wider than efac x the cell side at the first depth
+ * whose non-empty cells number at least (system size) / occ.  Defaults 64, 2,
+ * 32.  walk_blocks: persistent blocks of the mixed walk (0 = the device's
+ * occupancy limit). */
+int bh_set_group_params(bh_ctx *ctx, int gmax, double efac, int occ, int walk_blocks);
+int bh_get_group_params(bh_ctx *ctx, int *gmax, double *efac, int *occ, int *walk_blocks);
+
+/* Autotuner of the mixed walk: times `reps` force evaluations of `form` on
+ * the current state (ctx's theta, G, soft2; no kick or drift) for each
+ * candidate group shape (gmax 32/48/64 x efac 1/2/4) and walk occupancy
+ * (full, half), keeps the fastest and reports it.  A LEARNED request before
+ * the model is trained is tuned as MIXED with the previous-cost schedule.
+ * Call before the first step: the accelerations of the last candidate stay
+ * on the device. */
+typedef struct bh_tune_result {
+    int gmax;
+    double efac;
+    int occ;
+    int walk_blocks;
+    double best_ms;     /* median force evaluation of the choice */
+    double default_ms;  /* the same with the default shape */
+    int n_tried;
+} bh_tune_result;
+int bh_autotune(bh_ctx *ctx, int form, int reps, bh_tune_result *out);
 
 /* ------------------------------------------------ synthetic inputs --- */
 #define BH_GEN_BELT 0   /* central mass + 8 planets + main belt */

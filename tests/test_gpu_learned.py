@@ -114,6 +114,7 @@ def test_learned_gain_over_schedules_without_it(bh):
     for name, form, sched in (("key_order", bh.FORM_MIXED, bh.SCHED_KEY_ORDER),
                               ("prev_counts", bh.FORM_MIXED, bh.SCHED_PREV_COST)):
         with bh.Engine(form=form, theta=1.05) as e2:
+            e2.set_graph(False)   # eager steps record the phase events read below
             e2.set_schedule(sched)
             e2.set_bodies(state["pos"], state["vel"], state["mass"], state["ids"])
             e2.step(3, HOUR)
@@ -121,6 +122,7 @@ def test_learned_gain_over_schedules_without_it(bh):
             ph, k = e2.step_phase_ms()
             out[f"walk_ms_{name}"] = float(ph[3])
     with bh.Engine(form=bh.FORM_LEARNED, theta=1.05) as e3:
+        e3.set_graph(False)
         e3.set_bodies(pos, vel, m)
         e3.step(50, HOUR)
         e3.step(10, HOUR)
