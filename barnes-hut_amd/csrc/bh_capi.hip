@@ -319,10 +319,9 @@ int check_device_errors(bh_ctx* c)
     HIP_TRY(c, hipMemcpy(&err, &c->tb.sc->error, sizeof err, hipMemcpyDeviceToHost));
     if (err & bh::kErrStack) return fail(c, BH_ERR_STATE, "mixed walk: traversal stack capacity exceeded");
     if (err) return fail(c, BH_ERR_STATE, "device guard raised error bits 0x%x", err);
-    uint32_t chain[2] = {0, 0};
-    HIP_TRY(c, hipMemcpy(&chain[0], bh::scan_error_word(c->tb.scan_tmp), 4, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy(&chain[1], bh::sort_error_word(c->tb.sort_tmp), 4, hipMemcpyDeviceToHost));
-    if (chain[0] | chain[1]) return fail(c, BH_ERR_STATE, "scan/sort: a look-back wait gave up");
+    uint32_t chain = 0;
+    HIP_TRY(c, hipMemcpy(&chain, bh::sort_error_word(c->tb.sort_tmp), 4, hipMemcpyDeviceToHost));
+    if (chain) return fail(c, BH_ERR_STATE, "sort: a look-back wait gave up (0x%x)", chain);
     return BH_OK;
 }
 

@@ -86,10 +86,8 @@ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // ----------------------------------------------------------------- scan
 // Exclusive prefix sum of n u32 values; total (optional device pointer)
-// receives the sum.  tmp must hold scan_tmp_words(n) u32 (status words sit at
-// a fixed offset, so any smaller n may use the same tmp), zeroed once before
-// first use (it carries the look-back state from launch to launch); launches
-// that share a tmp must run in stream order.
+// receives the sum (in and out may alias).  tmp must hold scan_tmp_words(n)
+// u32; launches that share a tmp must run in stream order.
 size_t scan_tmp_words(size_t n);
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
                         hipStream_t s);
@@ -99,14 +97,13 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size
 // Input in (k0, v0); result in (k0, v0) after the call (ping-pong through
 // k1, v1).  tmp must hold sort_tmp_words(cap) u32 for every n <= cap, zeroed
 // once before first use, like the scan's.
-// scan_error_word / sort_error_word: device address of the word that becomes
-// nonzero if a look-back wait of a scan / sort using that tmp ever gave up.
+// sort_error_word: device address of the word that becomes nonzero if a
+// look-back wait of a sort using that tmp ever gave up.
 size_t sort_tmp_words(size_t cap);
 // With sysd (the system of each index; values < 256) a last stable pass
 // orders by sysd[payload], i.e. by (system, key).
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
                       const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s);
-uint32_t* scan_error_word(uint32_t* scan_tmp);
 uint32_t* sort_error_word(uint32_t* sort_tmp);
 
 // ----------------------------------------------------------------- step

@@ -668,10 +668,14 @@ __global__ __launch_bounds__(kNT) void k_heavy_partial(Bodies b, TreeBuffers tb,
     }
 }
 
-// One thread per (system, heavy rank, axis).
+// One wavefront per (system, heavy rank, axis): lane l adds the chunk
+// partials l, l + 64, ... (Neumaier), then the 64 lane sums are added in lane
+// order -- a fixed order for a given system.
 __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
 {
-    const uint32_t t = blockIdx.x * kNT + threadIdx.x;
+    __shared__ double lanes[kWaves][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * kWaves + w;
     const uint32_t a = t % 3, k = (t / 3) % kMaxHeavy, q = t / (3 * kMaxHeavy);
     if (q >= tb.n_sys) return;
     const SysScalars* sy = tb.sys + q;
@@ -679,8 +683,13 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
     const uint32_t h = sy->heavy_slot[k];
     Neumaier acc;
     const size_t first = sy->heavy_item + static_cast<size_t>(k) * sy->heavy_chunks;
-    for (uint32_t c = 0; c < sy->heavy_chunks; ++c) acc.add(mb.hpart[3 * (first + c) + a]);
-    const double v = acc.value();
+    for (uint32_t c = static_cast<uint32_t>(lane); c < sy->heavy_chunks; c += 64) acc.add(mb.hpart[3 * (first + c) + a]);
+    lanes[w][lane] = acc.value();
+    __builtin_amdgcn_wave_barrier();
+    if (lane != 0) return;
+    Neumaier tot;
+    for (int l = 0; l < 64; ++l) tot.add(lanes[w][l]);
+    const double v = tot.value();
     double* accel = a == 0 ? b.ax : (a == 1 ? b.ay : b.az);
     double* vel = a == 0 ? b.vx : (a == 1 ? b.vy : b.vz);
     accel[h] = v;
@@ -812,7 +821,7 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
     if (p.heavy_items > 0) {
         const unsigned hb = std::min<unsigned>((p.heavy_items + kWaves - 1) / kWaves, 4096u);
         k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
-        k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kNT - 1) / kNT, kNT, 0, s>>>(b, tb, mb, p);
+        k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kWaves - 1) / kWaves, kNT, 0, s>>>(b, tb, mb, p);
     }
 }
 

@@ -1,7 +1,7 @@
 // bh_sort.hip -- device-wide exclusive scan and a stable LSD radix sort of
 // 64-bit Morton keys with a 32-bit payload, both single-pass over their data.
 //
-// Tiles are chained by decoupled look-back: a block takes the next tile id
+// Sort tiles are chained by decoupled look-back: a block takes the next tile id
 // from an atomic counter, publishes its tile's aggregate, walks back over the
 // status words of the tiles before it until it finds an inclusive prefix, and
 // publishes its own inclusive prefix.  A status word is 64 bits -- tag
@@ -15,7 +15,9 @@
 // sets the chain's error word (read by the host after the step) instead of
 // spinning forever.
 //
-// Scan: tiles of 2048 values (256 threads x 8), one launch.
+// Scan: tiles of 2048 values (256 threads x 8), reduce-then-scan in three
+// launches (no look-back: at these sizes the chain's latency costs more
+// than reading the values twice).
 //
 // Sort: 8-bit digits.  One launch counts the digits of every pass (per-block
 // LDS histograms, reduced and scanned by the last block), then one launch
@@ -169,20 +171,55 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uin
     return r;
 }
 
-// Single-pass exclusive scan; in and out may alias (a tile reads its values
-// before it writes them).
-__global__ __launch_bounds__(kNT) void k_scan(const uint32_t* in, uint32_t* out, size_t n,
-                                              uint32_t* __restrict__ total_out, uint32_t* tmp)
+// Exclusive scan, reduce-then-scan: tile sums, a one-block scan of the tile
+// sums, then each tile's local scan plus its offset.  Deterministic and free
+// of inter-block waits; every value is read twice.
+__global__ __launch_bounds__(kNT) void k_scan_tile_sums(const uint32_t* __restrict__ in, size_t n,
+                                                        uint32_t* __restrict__ sums)
 {
     __shared__ uint32_t lds[kWaves + 1];
-    __shared__ uint32_t s_id, s_epoch, s_excl;
-    const Chain c = chain_at(tmp);
-    if (!chain_enter(c, &s_id, &s_epoch)) {
-        chain_leave(c, s_epoch);
-        return;
+    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kIPT; ++j)
+        if (base + j < n) s += in[base + j];
+    uint32_t total;
+    block_exclusive_scan(s, &total, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kNT) void k_scan_sums(uint32_t* __restrict__ sums, size_t nt,
+                                                   uint32_t* __restrict__ total_out)
+{
+    __shared__ uint32_t lds[kWaves + 1];
+    uint32_t carry = 0;
+    for (size_t base = 0; base < nt; base += kTile) {
+        const size_t b = base + static_cast<size_t>(threadIdx.x) * kIPT;
+        uint32_t v[kIPT];
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < kIPT; ++j) {
+            v[j] = (b + j < nt) ? sums[b + j] : 0u;
+            s += v[j];
+        }
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan(s, &total, lds) + carry;
+#pragma unroll
+        for (int j = 0; j < kIPT; ++j) {
+            if (b + j < nt) sums[b + j] = ex;
+            ex += v[j];
+        }
+        carry += total;
     }
-    const uint32_t tile = s_id, epoch = s_epoch;
-    const size_t base = static_cast<size_t>(tile) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+// in and out may alias: a tile reads its values before it writes them
+__global__ __launch_bounds__(kNT) void k_scan_apply(const uint32_t* in, size_t n, const uint32_t* __restrict__ sums,
+                                                    uint32_t* out)
+{
+    __shared__ uint32_t lds[kWaves + 1];
+    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + static_cast<size_t>(threadIdx.x) * kIPT;
     uint32_t v[kIPT];
     uint32_t s = 0;
 #pragma unroll
@@ -190,28 +227,13 @@ __global__ __launch_bounds__(kNT) void k_scan(const uint32_t* in, uint32_t* out,
         v[j] = (base + j < n) ? in[base + j] : 0u;
         s += v[j];
     }
-    uint32_t agg;
-    uint32_t ex = block_exclusive_scan(s, &agg, lds);
-    if (threadIdx.x == 0) {
-        uint32_t excl = 0;
-        if (tile == 0) {
-            publish(c.status, status_word(epoch, true, agg));
-        } else {
-            publish(c.status + tile, status_word(epoch, false, agg));
-            excl = look_back(c, tile, 1, 0, epoch);
-            publish(c.status + tile, status_word(epoch, true, excl + agg));
-        }
-        if (tile == gridDim.x - 1 && total_out) *total_out = excl + agg;
-        s_excl = excl;
-    }
-    __syncthreads();
-    ex += s_excl;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(s, &total, lds) + sums[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < kIPT; ++j) {
         if (base + j < n) out[base + j] = ex;
         ex += v[j];
     }
-    chain_leave(c, epoch);
 }
 
 // ------------------------------------------------------------ radix sort
@@ -377,22 +399,22 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
 
 }  // namespace
 
-size_t scan_tmp_words(size_t cap) { return kChainHeadWords + 2 * sort_tiles(cap); }
+size_t scan_tmp_words(size_t cap) { return sort_tiles(cap) + 1; }
 
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
                         hipStream_t s)
 {
     size_t nt = (n + kTile - 1) / kTile;
     if (nt == 0) nt = 1;
-    k_scan<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, out, n, total, tmp);
+    k_scan_tile_sums<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp);
+    k_scan_sums<<<1, kNT, 0, s>>>(tmp, nt, total);
+    k_scan_apply<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp, out);
 }
 
 size_t sort_tmp_words(size_t cap)
 {
     return 4 + 2 * kMaxPasses * kDigits + kChainHeadWords + 2 * kDigits * sort_tiles(cap);
 }
-
-uint32_t* scan_error_word(uint32_t* scan_tmp) { return scan_tmp + 3; }
 
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
 

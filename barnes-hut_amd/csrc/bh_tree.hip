@@ -291,7 +291,8 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
     if (i == 0) tb.leaf_first[tb.sc->n_leaves] = static_cast<uint32_t>(n);
 }
 
-// leaf raw moments; boundary depths and representatives
+// leaf moments (raw, or light moments and boxes for the mixed forms);
+// boundary depths and representatives
 __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n, TreeBuffers tb)
 {
     const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
@@ -299,22 +300,23 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
     const uint32_t M = tb.sc->n_leaves;
     tb.cnt_hi[j] = 0;
     if (j < M) {
-        // src/linear_octree.cpp:28-38: n.mass += m; n.com += x * m, from 0.0
-        double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
         const uint32_t e = tb.leaf_first[j + 1];
-        for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
-            const double mi = b.m[i];
-            sm += mi;
-            sx += b.px[i] * mi;
-            sy += b.py[i] * mi;
-            sz += b.pz[i] * mi;
-        }
-        double* r = tb.raw + 4 * j;
-        r[0] = sm;
-        r[1] = sx;
-        r[2] = sy;
-        r[3] = sz;
-        if (tb.lraw) {
+        if (!tb.lraw) {
+            // src/linear_octree.cpp:28-38: n.mass += m; n.com += x * m, from 0.0
+            double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+            for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
+                const double mi = b.m[i];
+                sm += mi;
+                sx += b.px[i] * mi;
+                sy += b.py[i] * mi;
+                sz += b.pz[i] * mi;
+            }
+            double* r = tb.raw + 4 * j;
+            r[0] = sm;
+            r[1] = sx;
+            r[2] = sy;
+            r[3] = sz;
+        } else {
             // moments without the heavy bodies, which the mixed walk sums directly
             double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
             double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -417,29 +419,31 @@ __global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
     tb.child[8 * static_cast<size_t>(pb) + oct] = static_cast<int32_t>(t);
 }
 
-// raw moments of branching node bi from its existing children, summed in
-// octant order 0..7 from 0.0 (src/linear_octree.cpp:69-87); the light
-// moments and light-body boxes alongside when the mixed forms need them
+// Moments of branching node bi from its existing children, summed in octant
+// order 0..7 from 0.0 (src/linear_octree.cpp:69-87): the raw moments for the
+// plain form, or, when the mixed forms set tb.lraw, the light moments and
+// light-body boxes (the mixed walk never reads the raw moments).
 __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi)
 {
-    double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
     const int32_t* c = tb.child + 8 * bi;
+    if (!tb.lraw) {
+        double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int32_t ck = c[k];
-        if (ck < 0) continue;
-        const double* r = tb.raw + 4 * static_cast<size_t>(ck);
-        sm += r[0];
-        sx += r[1];
-        sy += r[2];
-        sz += r[3];
-    }
-    double* r = tb.raw + 4 * (static_cast<size_t>(M) + bi);
-    r[0] = sm;
-    r[1] = sx;
-    r[2] = sy;
-    r[3] = sz;
-    if (tb.lraw) {
+        for (int k = 0; k < 8; ++k) {
+            const int32_t ck = c[k];
+            if (ck < 0) continue;
+            const double* r = tb.raw + 4 * static_cast<size_t>(ck);
+            sm += r[0];
+            sx += r[1];
+            sy += r[2];
+            sz += r[3];
+        }
+        double* r = tb.raw + 4 * (static_cast<size_t>(M) + bi);
+        r[0] = sm;
+        r[1] = sx;
+        r[2] = sy;
+        r[3] = sz;
+    } else {
         double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
         double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll

@@ -196,7 +196,7 @@ CONFIGS = {
     "c3": dict(kind="trojan", n=1048576, theta=1.05, steps=200, form="learned", forms="plain,mixed,learned",
                workload="belt 60% + Jupiter Trojans 2 x 15% + Hildas 10%, 1048576 bodies, theta 1.05, learned "
                         "schedule"),
-    "c4": dict(kind="belt", ensemble=64, nmin=10000, nmax=200000, theta=1.05, steps=100, form="learned",
+    "c4": dict(kind="belt", ensemble=64, nmin=10000, nmax=200000, theta=1.05, steps=100, form="mixed",
                forms="mixed,learned",
                workload="64 independent belt systems (seeds 0..63), N log-uniform in [1e4, 2e5], one batched "
                         "graph-replayed step, autotuned group shape, theta 1.05, dt 1 h"),
