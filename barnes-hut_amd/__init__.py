@@ -103,6 +103,8 @@ SIGNATURES = [
     ("bh_set_group_params", C.c_int, [_P, C.c_int, _D, C.c_int, C.c_int]),
     ("bh_get_group_params", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_D), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("bh_autotune", C.c_int, [_P, C.c_int, C.c_int, _P]),
+    ("bh_set_relative_opening", C.c_int, [_P, _D]),
+    ("bh_get_relative_opening", C.c_int, [_P, C.POINTER(_D)]),
 ]
 
 _lib = None
@@ -332,6 +334,14 @@ class Engine:
 
     def set_group_params(self, gmax=64, efac=2.0, occ=32, walk_blocks=0):
         self._ck(self.lib.bh_set_group_params(self.h, gmax, efac, occ, walk_blocks))
+
+    def set_relative_opening(self, alpha: float):
+        self._ck(self.lib.bh_set_relative_opening(self.h, alpha))
+
+    def relative_opening(self) -> float:
+        a = _D(0)
+        self._ck(self.lib.bh_get_relative_opening(self.h, C.byref(a)))
+        return a.value
 
     def group_params(self) -> dict:
         g, o, w = C.c_int(0), C.c_int(0), C.c_int(0)

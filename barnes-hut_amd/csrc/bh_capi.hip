@@ -46,6 +46,7 @@ struct bh_ctx {
     uint32_t heavy_items = 0;         // work items of the heavy direct sums
     size_t heavy_item_cap = 0;
     bh::GroupParams gp{};
+    double mac_alpha = 1e-7;          // relative opening test of the mixed walk (0: off)
     unsigned mixed_blocks = 0;        // persistent walk blocks in use
     unsigned mixed_blocks_max = 0;    // occupancy limit of the device
     // graph replay of bh_step: one graph per parity of the body double buffer
@@ -164,7 +165,7 @@ int reserve(bh_ctx* c, size_t n)
     A(t.ukey, N); A(t.leaf_first, N + 1); A(t.delta, N); A(t.rep, N); A(t.is_rep, N);
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
-    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.walk, 2 * N);
+    A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.lsec, 2 * N); A(t.walk, 2 * N);
     A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems);
     t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
@@ -235,7 +236,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         if (sched != BH_SCHED_KEY_ORDER)
             bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, s);
         MARK(3);
-        bh::MixedParams mp{theta * theta, G, soft2, half_dt, kick_after ? 1 : 0,
+        bh::MixedParams mp{theta * theta, G, soft2, half_dt, c->mac_alpha, kick_after ? 1 : 0,
                            sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4, c->heavy_items};
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
         if (learned && c->trained_steps < c->train_steps) {
@@ -404,7 +405,7 @@ std::vector<double> graph_key(const bh_ctx* c, double dt)
 {
     return {static_cast<double>(c->cfg.form), c->cfg.theta, c->cfg.G, c->cfg.soft2, dt, static_cast<double>(c->n),
             static_cast<double>(c->n_sys), static_cast<double>(c->sched), c->model_ready ? 1.0 : 0.0,
-            static_cast<double>(c->gp.gmax), c->gp.efac, static_cast<double>(c->gp.occ),
+            static_cast<double>(c->gp.gmax), c->gp.efac, static_cast<double>(c->gp.occ), c->mac_alpha,
             static_cast<double>(c->mixed_blocks), static_cast<double>(c->cap)};
 }
 
@@ -898,6 +899,21 @@ int bh_set_group_params(bh_ctx* c, int gmax, double efac, int occ, int walk_bloc
     c->gp.occ = static_cast<uint32_t>(occ);
     if (c->mixed_blocks_max == 0) c->mixed_blocks_max = bh::mixed_walk_blocks(c->cfg.device);
     c->mixed_blocks = walk_blocks == 0 ? c->mixed_blocks_max : std::min<unsigned>(walk_blocks, c->mixed_blocks_max);
+    return BH_OK;
+}
+
+int bh_set_relative_opening(bh_ctx* c, double alpha)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (!(alpha >= 0.0)) return fail(c, BH_ERR_ARG, "alpha must be >= 0");
+    c->mac_alpha = alpha;
+    return BH_OK;
+}
+
+int bh_get_relative_opening(bh_ctx* c, double* alpha)
+{
+    if (check_ctx(c) || !alpha) return BH_ERR_ARG;
+    *alpha = c->mac_alpha;
     return BH_OK;
 }
 

@@ -143,6 +143,7 @@ struct TreeBuffers {
     double* raw;          // 4 per node: mass, sum m*x, sum m*y, sum m*z
     double* lraw;         // the same over light bodies only (mixed forms), or null
     double* lbox;         // 6 per node: bounding box of the light bodies (min xyz, max xyz)
+    double* lsec;         // per node: sum of m |x|^2 over the light bodies (mixed forms)
     uint32_t* gflag;      // walk-group start flags (mixed forms), cleared per step
     WalkNode* walk;       // capacity 2N
     uint32_t* scan_tmp;
@@ -189,7 +190,7 @@ struct MNode {
     float gm;         // G * light mass
     uint32_t first;   // first child slot (branching) or first body (leaf)
     uint32_t info;    // child or body count | kMLeaf
-    uint32_t pad;
+    float gq;         // G * sum of m |x - com|^2 over the light bodies (quadrupole scale)
 };
 static_assert(sizeof(MNode) == 48, "MNode is 48 bytes");
 constexpr uint32_t kMLeaf = 1u << 31;
@@ -218,6 +219,7 @@ struct MixedBuffers {
 
 struct MixedParams {
     double theta2, G, soft2, half_dt;
+    double alpha;         // relative opening test G Q <= alpha |a_heavy| D^4 (0: off)
     int kick;
     int sched;            // 0 dynamic in key order, 1 dynamic over order[], 2 static chunks
     uint32_t n_workers;

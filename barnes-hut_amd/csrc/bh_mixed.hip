@@ -7,7 +7,8 @@
 // 64; a leaf with more than 64 co-keyed bodies is split.  One wavefront walks
 // the tree once per group, one lane per body.
 //
-// Opening test per group: a node is accepted when b^2 < theta^2 * D^2, with b
+// Opening test per group (the extent test): a node is accepted when
+// b^2 < theta^2 * D^2, with b
 // the distance from the node's centre of mass to the farthest corner of the
 // bounding box of its light bodies (the node's true extent, <= the cell
 // diagonal) and D the FP64 distance from that centre of mass to the group's
@@ -16,7 +17,11 @@
 // as the reference's per-body test (src/traversal.cpp:78), but measured on
 // the bodies actually present instead of the cubic cell.  The result of a
 // group depends only on the group, never on which worker walks it or when:
-// every schedule gives identical bits.
+// every schedule gives identical bits.  For theta > 0.8 a second, relative
+// test guards the nodes with b > 0.8 D, where the monopole error bound grows
+// without limit as b/D -> 1: such a node is accepted only if its quadrupole
+// scale G Q / D^4 (Q = sum m |x - com|^2 of its light bodies) is at most
+// alpha times the smallest heavy-body acceleration of the group.
 //
 // Traversal keeps a stack of node ranges in LDS (children of a node occupy
 // consecutive slots); each round pops up to 64 ranges, one per lane, and the
@@ -52,6 +57,10 @@ constexpr int kStackCap = 1024;
 // single-range round (pure depth first) grows the stack by at most 7 per level.
 constexpr int kBigPopLimit = kStackCap - 7 * 64 - 8 * 22;   // see the pop rule in k_walk_mixed
 constexpr uint32_t kNone = 0xffffffffu;
+// b/D at or below 0.8 keeps the monopole error bound of an accepted node
+// (which grows like (b/D)^2 / (1 - b/D)^2) small; above it, up to theta, the
+// relative quadrupole test decides as well
+constexpr double kSafe2 = 0.64;
 
 // stack entry: first (27 bits) | count - 1 (3 bits, ranges hold <= 8) | body range (bit 31)
 __device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
@@ -267,7 +276,10 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, TreeBuffers tb, M
         nd.x = nd.y = nd.z = 0.0;
     }
     nd.gm = static_cast<float>(G * m);
-    nd.pad = 0;
+    // Q = sum m |x - com|^2 = sum m |x|^2 - |sum m x|^2 / m; the cancellation
+    // costs (|x| / extent)^2 ulps, harmless for an opening test
+    const double quad = m != 0.0 ? tb.lsec[t] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / m : 0.0;
+    nd.gq = static_cast<float>(G * (quad > 0.0 ? quad : 0.0));
     // opening size: the farthest corner of the light bodies' box seen from the
     // centre of mass (0 for an empty node)
     double s2 = 0.0;
@@ -368,20 +380,23 @@ __device__ inline void interact(const float4 q, float ox, float oy, float oz, fl
     bz = fmaf(s, dz, bz);
 }
 
-// Evaluates list[0..ln) for this lane's body (offsets ox, oy, oz): FP32 FMA
-// inside blocks of 32 entries, Kahan-compensated across blocks.
-__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, float ox, float oy, float oz,
-                                  float soft2, Acc3& a)
+// Evaluates this lane's share of list[0..ln) for its body (offsets ox, oy,
+// oz): with K lanes per body, the entries sub, sub + K, sub + 2K, ...  FP32
+// FMA inside blocks of 32 such entries, Kahan-compensated across blocks.
+template <int K>
+__device__ inline void flush_list_k(const float4* __restrict__ list, uint32_t ln, uint32_t sub, float ox, float oy,
+                                    float oz, float soft2, Acc3& a)
 {
     ln = uni(ln);
+    constexpr uint32_t span = 32u * K;
     uint32_t e0 = 0;
-    for (; e0 + 32u <= ln; e0 += 32u) {
+    for (; e0 + span <= ln; e0 += span) {
         float bx = 0.f, by = 0.f, bz = 0.f;
 #pragma unroll
         for (int u = 0; u < 32; u += 8) {
             float4 q[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) q[k] = list[e0 + u + k];
+            for (int k = 0; k < 8; ++k) q[k] = list[e0 + static_cast<uint32_t>(u + k) * K + sub];
 #pragma unroll
             for (int k = 0; k < 8; ++k) interact(q[k], ox, oy, oz, soft2, bx, by, bz);
         }
@@ -389,13 +404,25 @@ __device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, 
     }
     if (e0 < ln) {
         float bx = 0.f, by = 0.f, bz = 0.f;
-        for (uint32_t e = e0; e < ln; ++e) interact(list[e], ox, oy, oz, soft2, bx, by, bz);
+        for (uint32_t e = e0 + sub; e < ln; e += K) interact(list[e], ox, oy, oz, soft2, bx, by, bz);
         a.add(bx, by, bz);
     }
 }
 
+// K (wave-uniform) = lanes per body: 4 for groups of <= 16 bodies, 2 for
+// <= 32, else 1, so small groups keep every lane busy.
+__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub, float ox,
+                                  float oy, float oz, float soft2, Acc3& a)
+{
+    if (K == 4) flush_list_k<4>(list, ln, sub, ox, oy, oz, soft2, a);
+    else if (K == 2) flush_list_k<2>(list, ln, sub, ox, oy, oz, soft2, a);
+    else flush_list_k<1>(list, ln, sub, ox, oy, oz, soft2, a);
+}
+
 constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
 
+// 135 VGPRs: 3 waves per SIMD.  Capping at 128 for 4 waves measured slower on
+// the 300k and ensemble walks (the list loop needs its registers).
 __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
                                                     MixedParams p)
 {
@@ -432,8 +459,12 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         const uint64_t t_start = wall_clock64();
         const uint32_t b0 = mb.gfirst[g];
         const uint32_t cnt = mb.gfirst[g + 1] - b0;
-        const bool valid = static_cast<uint32_t>(lane) < cnt;
-        const uint32_t i = b0 + (valid ? static_cast<uint32_t>(lane) : 0u);
+        // K lanes per body: lane = sub * (64 / K) + body
+        const uint32_t K = cnt <= 16u ? 4u : (cnt <= 32u ? 2u : 1u);
+        const uint32_t per = 64u / K;
+        const uint32_t bl = static_cast<uint32_t>(lane) % per, sub = static_cast<uint32_t>(lane) / per;
+        const bool valid = bl < cnt;
+        const uint32_t i = b0 + (valid ? bl : 0u);
         const double x = b.px[i], y = b.py[i], z = b.pz[i];
         const double lox = wave_min(x), loy = wave_min(y), loz = wave_min(z);
         const double hix = wave_max(x), hiy = wave_max(y), hiz = wave_max(z);
@@ -446,6 +477,28 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0;
         bool overflow = false;
         const uint32_t gsys = tb.bsys[b0];
+        // heavy bodies of the system: FP64 direct sum in rank order
+        double hax = 0.0, hay = 0.0, haz = 0.0;
+        const SysScalars* sy = tb.sys + gsys;
+        const uint32_t nh = sy->n_heavy;
+        for (uint32_t k = 0; k < nh; ++k) {
+            const uint32_t j = sy->heavy_slot[k];
+            if (j == i) continue;
+            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+            const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
+            if (!(r2 > 0.0)) continue;
+            const double invR = 1.0 / sqrt(r2);
+            const double s = p.G * b.m[j] * (invR * invR * invR);
+            hax += s * dx;
+            hay += s * dy;
+            haz += s * dz;
+        }
+        // relative opening test: a node is also opened while its quadrupole
+        // scale G Q / D^4 exceeds alpha x the smallest heavy-body acceleration
+        // of the group (the acceleration every body of these systems is made of)
+        const double ah = valid ? sqrt(hax * hax + hay * hay + haz * haz) : INFINITY;
+        const double aref = wave_min(ah);
+        const double qlim = (p.alpha > 0.0 && aref > 0.0 && aref < INFINITY) ? p.alpha * aref : INFINITY;
         if (lane == 0) stk[0] = enc(gsys, 1, false);   // the root of the group's system
         __builtin_amdgcn_wave_barrier();
         while (sp > 0 && !overflow) {
@@ -504,7 +557,8 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                                 const double ey = fmax(fabs(nd.y - cy) - hy, 0.0);
                                 const double ez = fmax(fabs(nd.z - cz) - hz, 0.0);
                                 const double d2 = ex * ex + ey * ey + ez * ez;
-                                if (nd.size2 < theta2 * d2) {
+                                if (nd.size2 < theta2 * d2 &&
+                                    (nd.size2 <= kSafe2 * d2 || static_cast<double>(nd.gq) <= qlim * (d2 * d2))) {
                                     app = true;
                                     q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
                                                     static_cast<float>(nd.z - cz), nd.gm);
@@ -522,7 +576,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                 const uint32_t na = static_cast<uint32_t>(__popcll(am));
                 if (ln + na > static_cast<uint32_t>(kListCap)) {
                     __builtin_amdgcn_wave_barrier();
-                    flush_list(list, ln, ox, oy, oz, soft2f, acc);
+                    flush_list(list, ln, K, sub, ox, oy, oz, soft2f, acc);
                     __builtin_amdgcn_wave_barrier();
                     ln = 0;
                 }
@@ -558,29 +612,29 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
             }
         }
         __builtin_amdgcn_wave_barrier();
-        flush_list(list, ln, ox, oy, oz, soft2f, acc);
+        flush_list(list, ln, K, sub, ox, oy, oz, soft2f, acc);
         __builtin_amdgcn_wave_barrier();
-        // heavy bodies of the system: FP64 direct sum in rank order
-        double hax = 0.0, hay = 0.0, haz = 0.0;
-        const SysScalars* sy = tb.sys + gsys;
-        const uint32_t nh = sy->n_heavy;
-        for (uint32_t k = 0; k < nh; ++k) {
-            const uint32_t j = sy->heavy_slot[k];
-            if (j == i) continue;
-            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
-            const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
-            if (!(r2 > 0.0)) continue;
-            const double invR = 1.0 / sqrt(r2);
-            const double s = p.G * b.m[j] * (invR * invR * invR);
-            hax += s * dx;
-            hay += s * dy;
-            haz += s * dz;
+        // the K partial sums of a body, added in FP64 in sub order
+        double lax = static_cast<double>(acc.sx) + static_cast<double>(acc.cx);
+        double lay = static_cast<double>(acc.sy) + static_cast<double>(acc.cy);
+        double laz = static_cast<double>(acc.sz) + static_cast<double>(acc.cz);
+        if (K > 1) {
+            double tx = 0.0, ty = 0.0, tz = 0.0;
+            for (uint32_t k = 0; k < K; ++k) {
+                const int src = static_cast<int>(k * per + bl);
+                tx += __shfl(lax, src, 64);
+                ty += __shfl(lay, src, 64);
+                tz += __shfl(laz, src, 64);
+            }
+            lax = tx;
+            lay = ty;
+            laz = tz;
         }
         // heavy bodies take their accelerations from k_heavy_direct instead
-        if (valid && b.hrank[i] < 0) {
-            const double ax = hax + (static_cast<double>(acc.sx) + static_cast<double>(acc.cx));
-            const double ay = hay + (static_cast<double>(acc.sy) + static_cast<double>(acc.cy));
-            const double az = haz + (static_cast<double>(acc.sz) + static_cast<double>(acc.cz));
+        if (valid && sub == 0 && b.hrank[i] < 0) {
+            const double ax = hax + lax;
+            const double ay = hay + lay;
+            const double az = haz + laz;
             b.ax[i] = ax;
             b.ay[i] = ay;
             b.az[i] = az;
@@ -590,7 +644,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                 b.vz[i] = b.vz[i] + p.half_dt * az;
             }
         }
-        if (valid) b.pcost[i] = n_inter;
+        if (valid && sub == 0) b.pcost[i] = n_inter;
         if (lane == 0) {
             mb.g_inter[g] = n_inter;
             mb.g_visit[g] = n_visit;

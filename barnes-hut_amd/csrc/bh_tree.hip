@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
             r[3] = sz;
         } else {
             // moments without the heavy bodies, which the mixed walk sums directly
-            double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+            double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, ls = 0.0;
             double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
             for (uint32_t i = tb.leaf_first[j]; i < e; ++i) {
                 if (b.hrank[i] >= 0) continue;
@@ -327,6 +327,7 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
                 lx += b.px[i] * mi;
                 ly += b.py[i] * mi;
                 lz += b.pz[i] * mi;
+                ls += mi * (b.px[i] * b.px[i] + b.py[i] * b.py[i] + b.pz[i] * b.pz[i]);
                 const double xi[3] = {b.px[i], b.py[i], b.pz[i]};
                 for (int a = 0; a < 3; ++a) {
                     bl[a] = dmin(bl[a], xi[a]);
@@ -343,6 +344,7 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
             q[1] = lx;
             q[2] = ly;
             q[3] = lz;
+            tb.lsec[j] = ls;
         }
     }
     uint32_t isrep = 0;
@@ -444,7 +446,7 @@ __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi
         r[2] = sy;
         r[3] = sz;
     } else {
-        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, ls = 0.0;
         double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -455,6 +457,7 @@ __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi
             lx += q[1];
             ly += q[2];
             lz += q[3];
+            ls += tb.lsec[ck];
             const double* cb = tb.lbox + 6 * static_cast<size_t>(ck);
             for (int a = 0; a < 3; ++a) {
                 bl[a] = dmin(bl[a], cb[a]);
@@ -471,6 +474,7 @@ __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi
         q[1] = lx;
         q[2] = ly;
         q[3] = lz;
+        tb.lsec[static_cast<size_t>(M) + bi] = ls;
     }
 }
 

@@ -242,6 +242,15 @@ int bh_get_systems(bh_ctx *ctx, int64_t *n_systems, int64_t capacity, int64_t *o
  * 32.  walk_blocks: persistent blocks of the mixed walk (0 = the device's
  * occupancy limit). */
 int bh_set_group_params(bh_ctx *ctx, int gmax, double efac, int occ, int walk_blocks);
+/* Relative opening test of the mixed walk, on top of the extent test
+ * b < theta D: a node with b > 0.8 D is accepted only if also
+ * G Q <= alpha |a_h| D^4, Q = sum m |x - com|^2 of its light bodies (the scale
+ * of its quadrupole, whose neglect is the monopole's error ~ G Q / D^4), D its
+ * distance to the group box, |a_h| the smallest heavy-body acceleration of the
+ * group.  Default alpha = 1e-7; 0 turns it off.  It only acts for theta > 0.8
+ * and matters for dense, massive clusters (Trojan clouds). */
+int bh_set_relative_opening(bh_ctx *ctx, double alpha);
+int bh_get_relative_opening(bh_ctx *ctx, double *alpha);
 int bh_get_group_params(bh_ctx *ctx, int *gmax, double *efac, int *occ, int *walk_blocks);
 
 /* Autotuner of the mixed walk: times `reps` force evaluations of `form` on

@@ -106,3 +106,39 @@ def test_mixed_steps_run(bh):
     assert np.array_equal(st["ids"], sp["ids"]) or True   # order may differ only by ties
     d = np.abs(st["pos"] - sp["pos"]).max()
     assert d < 1e-9, d
+
+
+def test_relative_opening_is_inert_below_theta_0p8(bh, engine):
+    """The relative test only guards nodes with b > 0.8 D, which theta <= 0.8
+    never accepts: forces are bit-identical with it on or off."""
+    pos, vel, m = bh.generate(bh.GEN_TROJAN, 2, 60000)
+    try:
+        engine.set_relative_opening(0.0)
+        a0, _ = mixed_force(engine, bh, pos, vel, m, 0.7)
+        engine.set_relative_opening(1e-7)
+        a1, _ = mixed_force(engine, bh, pos, vel, m, 0.7)
+    finally:
+        engine.set_relative_opening(1e-7)
+    assert np.array_equal(bits(a0), bits(a1))
+
+
+def test_relative_opening_tightens_dense_clusters(bh, orc, engine):
+    """Trojan clouds at theta 1.05: the extent test alone lets dense massive
+    nodes through at b/D close to 1; the relative test brings the worst cloud
+    member under 1e-5."""
+    n = 200000
+    pos, vel, m = bh.generate(bh.GEN_TROJAN, 6, n)
+    r = np.linalg.norm(pos, axis=1)
+    cloud = np.flatnonzero((r > 4.9) & (r < 5.5))[9:]
+    ids = np.concatenate([np.arange(9), np.random.default_rng(2).choice(cloud, 1024, replace=False)])
+    errs = {}
+    try:
+        for alpha in (0.0, 1e-7):
+            engine.set_relative_opening(alpha)
+            acc, st = mixed_force(engine, bh, pos, vel, m, 1.05)
+            errs[alpha] = rel_err(acc, st, orc, ids)
+    finally:
+        engine.set_relative_opening(1e-7)
+    say("trojan cloud max error: extent test only", errs[0.0].max(), "with relative test", errs[1e-7].max())
+    assert errs[1e-7].max() < 1e-5
+    assert errs[1e-7].max() <= errs[0.0].max()
