@@ -104,6 +104,7 @@ SIGNATURES = [
     ("bh_get_group_params", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_D), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("bh_autotune", C.c_int, [_P, C.c_int, C.c_int, _P]),
     ("bh_set_relative_opening", C.c_int, [_P, _D]),
+    ("bh_set_schedule_split", C.c_int, [_P, C.c_int]),
     ("bh_get_relative_opening", C.c_int, [_P, C.POINTER(_D)]),
 ]
 
@@ -310,6 +311,9 @@ class Engine:
 
     def set_schedule(self, mode: int):
         self._ck(self.lib.bh_set_schedule(self.h, mode))
+
+    def set_schedule_split(self, chunks_per_worker: int):
+        self._ck(self.lib.bh_set_schedule_split(self.h, chunks_per_worker))
 
     # ------------------------------------------------ graphs, ensembles, tuning
     def set_graph(self, on: bool):

@@ -56,6 +56,7 @@ struct bh_ctx {
     std::vector<double> gkey[2];      // launch parameters each graph was captured with
     int64_t graph_replays = 0;
     int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
+    int split = 0;                    // chunks per worker of the split schedule (0: heaviest first)
     bool model_ready = false;         // the LEARNED cost model is trained
     // cost model (allocated once per context, independent of N)
     ModelBuffers md{};
@@ -227,19 +228,29 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         bh::launch_mixed_prepare(c->n, tb, c->mb, c->gp, G, s);
         int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;
         const bool learned = form == BH_FORM_LEARNED;
+        const bool training = learned && c->trained_steps < c->train_steps;
+        const uint32_t n_workers = c->mixed_blocks * 4;
+        const uint32_t n_chunks = std::min<uint32_t>(static_cast<uint32_t>(c->split) * n_workers, 65535u);
+        const bool split = c->split > 0 && sched != BH_SCHED_KEY_ORDER;
+        // above this many groups the walk serves key order and the order
+        // kernels skip their work (training still needs the features)
+        const uint32_t order_max = split ? 0xffffffffu : 16u * n_workers;
         if (learned || sched == BH_SCHED_MODEL) {
-            bh::launch_group_features(c->n, c->cur, tb, c->mb, c->md.feat, s);
+            bh::launch_group_features(c->n, c->cur, tb, c->mb, c->md.feat, training ? 0xffffffffu : order_max, s);
             c->have_features = true;
         }
         if (sched == BH_SCHED_MODEL && !c->model_ready) sched = BH_SCHED_PREV_COST;
-        if (sched == BH_SCHED_MODEL) bh::launch_model_cost(c->n, c->md.feat, c->md, tb, c->mb, s);
-        if (sched != BH_SCHED_KEY_ORDER)
-            bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, s);
+        if (sched == BH_SCHED_MODEL) bh::launch_model_cost(c->n, c->md.feat, c->md, tb, c->mb, order_max, s);
+        if (split)
+            bh::launch_chunks(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, n_chunks, s);
+        else if (sched != BH_SCHED_KEY_ORDER)
+            bh::launch_schedule(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, order_max, s);
         MARK(3);
         bh::MixedParams mp{theta * theta, G, soft2, half_dt, c->mac_alpha, kick_after ? 1 : 0,
-                           sched == BH_SCHED_KEY_ORDER ? 0 : 1, c->mixed_blocks * 4, c->heavy_items};
+                           sched == BH_SCHED_KEY_ORDER ? 0 : (split ? 2 : 1), n_chunks, order_max,
+                           c->heavy_items};
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
-        if (learned && c->trained_steps < c->train_steps) {
+        if (training) {
             // training phase: the walk's list lengths become samples, then
             // iters_per_step Adam updates on mini-batches of all samples so far
             bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
@@ -406,6 +417,7 @@ std::vector<double> graph_key(const bh_ctx* c, double dt)
     return {static_cast<double>(c->cfg.form), c->cfg.theta, c->cfg.G, c->cfg.soft2, dt, static_cast<double>(c->n),
             static_cast<double>(c->n_sys), static_cast<double>(c->sched), c->model_ready ? 1.0 : 0.0,
             static_cast<double>(c->gp.gmax), c->gp.efac, static_cast<double>(c->gp.occ), c->mac_alpha,
+            static_cast<double>(c->split),
             static_cast<double>(c->mixed_blocks), static_cast<double>(c->cap)};
 }
 
@@ -998,6 +1010,14 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
         out->default_ms = def_ms;
         out->n_tried = tried;
     }
+    return BH_OK;
+}
+
+int bh_set_schedule_split(bh_ctx* c, int chunks_per_worker)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (chunks_per_worker < 0 || chunks_per_worker > 64) return fail(c, BH_ERR_ARG, "0..64 chunks per worker");
+    c->split = chunks_per_worker;
     return BH_OK;
 }
 

@@ -76,6 +76,7 @@ struct StepScalars {
     uint32_t error;          // kErr* bits raised by bounds guards
     uint32_t n_groups;       // walk groups of the mixed forms
     uint32_t group_counter;  // dynamic work queue of the mixed walk
+    uint32_t cost_total;     // sum of the group weights of the split schedule
 };
 
 constexpr uint32_t kErrWalkIndex = 1u;   // a walk index fell outside [0, M+B)
@@ -210,7 +211,7 @@ struct MixedBuffers {
     uint64_t* g_t1;     // N: wall clock when it finished
     uint32_t* g_worker; // N: worker wave that walked the group
     uint32_t* order;    // N: group order of the scheduled walk
-    uint32_t* chunk;    // workers + 1: static chunk starts into order[]
+    uint32_t* chunk;    // split schedule: first group of each chunk (n_chunks + 1, <= 65536)
     float* gcost;       // N: estimated cost of each group
     double* hpart;      // heavy direct sums: 3 partials per work item (system, heavy rank, chunk)
     uint32_t* sched_hist;    // 64 cost buckets (left at zero)
@@ -221,8 +222,9 @@ struct MixedParams {
     double theta2, G, soft2, half_dt;
     double alpha;         // relative opening test G Q <= alpha |a_heavy| D^4 (0: off)
     int kick;
-    int sched;            // 0 dynamic in key order, 1 dynamic over order[], 2 static chunks
-    uint32_t n_workers;
+    int sched;            // 0 dynamic in key order, 1 dynamic over order[], 2 dynamic over chunk[]
+    uint32_t n_chunks;    // chunks of the split schedule
+    uint32_t order_max_groups;   // sched 1 falls back to key order above this many groups
     uint32_t heavy_items; // work items of the heavy direct sums over all systems
 };
 
@@ -244,8 +246,14 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
 unsigned mixed_walk_blocks(int device);
 // Heaviest-first group order into mb.order (source 0: previous step's list
 // lengths; 1: mb.gcost from the cost model).
+// Kernels skip their work when the step has more than max_groups groups (the
+// walk then serves groups in key order).
 void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
-                     hipStream_t s);
+                     uint32_t max_groups, hipStream_t s);
+// Split schedule: n_chunks (<= 65535) runs of consecutive groups with equal
+// estimated cost (source as above) into mb.chunk.
+void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
+                   uint32_t n_chunks, hipStream_t s);
 
 // ------------------------------------------------------- cost model
 constexpr int kModelFeatures = 8;
@@ -269,9 +277,9 @@ struct ModelBuffers {
 constexpr int kMaxTrainBatch = 1 << 16;   // samples per gradient evaluation
 
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
-                           hipStream_t s);
+                           uint32_t max_groups, hipStream_t s);
 void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
-                       const MixedBuffers& mb, hipStream_t s);
+                       const MixedBuffers& mb, uint32_t max_groups, hipStream_t s);
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s);
 void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, const MixedBuffers& mb,
                            const ModelBuffers& md, hipStream_t s);

@@ -437,22 +437,32 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
     const uint32_t worker = blockIdx.x * kWaves + w;
     const float soft2f = static_cast<float>(p.soft2);
     const double theta2 = p.theta2;
-    uint32_t pos = 0, end = 0;
-    if (p.sched == 2) {
-        pos = worker < p.n_workers ? mb.chunk[worker] : 0;
-        end = worker < p.n_workers ? mb.chunk[worker + 1] : 0;
-    }
+    uint32_t pos = 0, end = 0;   // the current chunk's groups (sched 2)
+    // heaviest-first order pays while the tail decides the walk (few groups
+    // per worker); with many groups per worker the key order's locality wins
+    // (measured crossover between 8 and 33 groups per worker wave)
+    const bool use_order = p.sched == 1 && G <= p.order_max_groups;
     for (;;) {
         uint32_t g;
         if (p.sched == 2) {
+            // balanced chunks of consecutive groups, served by a dynamic queue;
+            // every take consumes a chunk, so the loop ends after n_chunks takes
+            while (pos >= end) {
+                uint32_t q = 0;
+                if (lane == 0) q = atomicAdd(&tb.sc->group_counter, 1u);
+                q = uni(q);
+                if (q >= p.n_chunks) break;
+                pos = uni(mb.chunk[q]);
+                end = uni(mb.chunk[q + 1]);
+            }
             if (pos >= end) break;
-            g = mb.order[pos++];
+            g = pos++;
         } else {
             uint32_t q = 0;
             if (lane == 0) q = atomicAdd(&tb.sc->group_counter, 1u);
             q = uni(q);
             if (q >= G) break;
-            g = (p.sched == 1) ? mb.order[q] : q;
+            g = use_order ? mb.order[q] : q;
         }
         g = uni(g);
         if (g >= G) continue;   // a schedule entry outside the groups of this step
@@ -771,10 +781,12 @@ constexpr unsigned kSchedBlocks = 256;
 // Source 0: one wavefront per group takes the largest previous list length of
 // its bodies.  Then every block counts its groups per bucket in LDS and adds
 // the counts to the global histogram once.
-__global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, int source)
+__global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, int source,
+                                                    uint32_t max_groups)
 {
     __shared__ uint32_t hist[64];
     const uint32_t G = tb.sc->n_groups;
+    if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 64) hist[threadIdx.x] = 0;
     if (source == 0) {
@@ -807,11 +819,12 @@ __global__ __launch_bounds__(64) void k_sched_scan(MixedBuffers mb)
 
 // Every block places a tile of 256 groups: LDS counters rank the tile's groups
 // inside their buckets, one global atomic per bucket reserves the tile's range.
-__global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb)
+__global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb, uint32_t max_groups)
 {
     __shared__ uint32_t cnt[64];
     __shared__ uint32_t base[64];
     const uint32_t G = tb.sc->n_groups;
+    if (G > max_groups) return;
     for (uint32_t t0 = blockIdx.x * kNT; t0 < G; t0 += gridDim.x * kNT) {
         if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
         __syncthreads();
@@ -829,9 +842,63 @@ __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb,
     }
 }
 
+// Split schedule.  Group weights: the estimated cost in units of 8 list
+// entries, plus one (so empty estimates still count), at most 65535.
+// Entries from n_groups to n are zero, so a scan over n adds the groups only.
+__global__ __launch_bounds__(kNT) void k_chunk_weights(size_t n, TreeBuffers tb, MixedBuffers mb,
+                                                       uint32_t* __restrict__ w)
+{
+    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (g >= n) return;
+    if (g >= tb.sc->n_groups) {
+        w[g] = 0;
+        return;
+    }
+    const float c = fmaxf(mb.gcost[g], 0.f) * 0.125f + 1.f;
+    w[g] = c >= 65535.f ? 65535u : static_cast<uint32_t>(c);
+}
+
+// chunk[c] = first group whose exclusive weight prefix reaches c / C of the
+// total, so chunk c covers the groups whose weight starts in its C-th of the
+// total; chunk[0] = 0, chunk[C] = G.
+__global__ __launch_bounds__(kNT) void k_chunk_bounds(TreeBuffers tb, MixedBuffers mb,
+                                                      const uint32_t* __restrict__ prefix, uint32_t n_chunks)
+{
+    const uint32_t c = blockIdx.x * kNT + threadIdx.x;
+    if (c > n_chunks) return;
+    const uint32_t G = tb.sc->n_groups;
+    if (c == 0 || c == n_chunks) {
+        mb.chunk[c] = c == 0 ? 0u : G;
+        return;
+    }
+    const uint64_t target = static_cast<uint64_t>(c) * tb.sc->cost_total / n_chunks;
+    uint32_t lo = 0, len = G;   // first g with prefix[g] >= target
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        if (prefix[lo + half] < target) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    mb.chunk[c] = lo;
+}
+
 inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kNT > 0 ? (n + kNT - 1) / kNT : 1); }
 
 }  // namespace
+
+void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
+                   uint32_t n_chunks, hipStream_t s)
+{
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0, 0xffffffffu);   // previous list lengths
+    // weights into mb.order, their exclusive prefix into mb.base (both free
+    // once the node array is built)
+    k_chunk_weights<<<nblk(n), kNT, 0, s>>>(n, tb, mb, mb.order);
+    scan_exclusive_u32(mb.order, mb.base, &tb.sc->cost_total, n, tb.scan_tmp, s);
+    k_chunk_bounds<<<(n_chunks + kNT) / kNT, kNT, 0, s>>>(tb, mb, mb.base, n_chunks);
+}
 
 void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, double G,
                           hipStream_t s)
@@ -859,12 +926,12 @@ unsigned mixed_walk_blocks(int device)
 }
 
 void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
-                     hipStream_t s)
+                     uint32_t max_groups, hipStream_t s)
 {
-    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0);   // previous list lengths
-    k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 1);                    // bucket histogram
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0, max_groups);   // previous list lengths
+    k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 1, max_groups);                    // bucket histogram
     k_sched_scan<<<1, 64, 0, s>>>(mb);
-    k_sched_scatter<<<kSchedBlocks, kNT, 0, s>>>(n, tb, mb);
+    k_sched_scatter<<<kSchedBlocks, kNT, 0, s>>>(n, tb, mb, max_groups);
 }
 
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,

@@ -87,9 +87,10 @@ __device__ inline T wave_reduce(T v, Op op)
 
 // One wavefront per group (grid-stride): lanes hold the group's bodies.
 __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
-                                                        float* __restrict__ feat)
+                                                        float* __restrict__ feat, uint32_t max_groups)
 {
     const uint32_t G = tb.sc->n_groups;
+    if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (kNT / 64);
     auto dmin = [](double a, double c) { return c < a ? c : a; };
@@ -128,9 +129,10 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
 // ------------------------------------------------------------ inference
 // gcost = 2^(16 y) - 1, the predicted list length of each group.
 __global__ __launch_bounds__(kNT) void k_model_cost(const float* __restrict__ feat, const uint16_t* __restrict__ wb,
-                                                    TreeBuffers tb, MixedBuffers mb)
+                                                    TreeBuffers tb, MixedBuffers mb, uint32_t max_groups)
 {
     __shared__ float P[kModelParams];
+    if (tb.sc->n_groups > max_groups) return;   // uniform: the whole grid returns
     load_params(wb, P);
     const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (g >= tb.sc->n_groups) return;
@@ -342,15 +344,15 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 }  // namespace
 
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
-                           hipStream_t s)
+                           uint32_t max_groups, hipStream_t s)
 {
-    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat);
+    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups);
 }
 
 void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
-                       const MixedBuffers& mb, hipStream_t s)
+                       const MixedBuffers& mb, uint32_t max_groups, hipStream_t s)
 {
-    k_model_cost<<<nblk(n), kNT, 0, s>>>(feat, md.wbf, tb, mb);
+    k_model_cost<<<nblk(n), kNT, 0, s>>>(feat, md.wbf, tb, mb, max_groups);
 }
 
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s)

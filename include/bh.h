@@ -160,6 +160,14 @@ int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
 #define BH_SCHED_PREV_COST 1
 #define BH_SCHED_MODEL 2
 int bh_set_schedule(bh_ctx *ctx, int mode);
+/* How PREV_COST and MODEL use the cost estimates: 0 = the dynamic queue
+ * serves groups heaviest first (longest processing time first) while there
+ * are at most 16 groups per worker wave, and in key order above that (the
+ * tail no longer decides the walk, locality does); k > 0 = the
+ * groups, in key order, are split into k x (worker waves) runs of equal
+ * estimated cost that the dynamic queue serves in key order, which keeps
+ * neighbouring groups on the device at the same time.  Default 0. */
+int bh_set_schedule_split(bh_ctx *ctx, int chunks_per_worker);
 
 /* ------------------------------------------------ learned cost model ---
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
@@ -198,7 +206,10 @@ int bh_model_train(bh_ctx *ctx, int64_t m, const float *features, const float *t
                    float lr, float *final_loss);
 /* ready: model in use; samples: collected; updates: Adam steps so far. */
 int bh_model_status(bh_ctx *ctx, int *ready, int64_t *samples, int64_t *updates, float *last_loss);
-/* Features of the groups of the last LEARNED evaluation (G x 8). */
+/* Features of the groups of the last LEARNED evaluation (G x 8).  After the
+ * training phase they are only computed while the walk uses the predicted
+ * order, i.e. with at most 16 groups per worker wave (see
+ * bh_set_schedule_split). */
 int bh_get_group_features(bh_ctx *ctx, int64_t capacity, float *features);
 
 /* Walk groups of the last MIXED / LEARNED evaluation: up to 64

@@ -62,12 +62,14 @@ def _trojan(bh, n, seed=3):
 def test_learned_forces_bit_identical_to_mixed(bh):
     pos, vel, m = _trojan(bh, 60000)
     res = {}
-    for name, form, sched in (("mixed", bh.FORM_MIXED, bh.SCHED_KEY_ORDER),
-                              ("mixed_prev", bh.FORM_MIXED, bh.SCHED_PREV_COST),
-                              ("learned", bh.FORM_LEARNED, bh.SCHED_MODEL)):
+    for name, form, sched, split in (("mixed", bh.FORM_MIXED, bh.SCHED_KEY_ORDER, 0),
+                                     ("mixed_prev", bh.FORM_MIXED, bh.SCHED_PREV_COST, 0),
+                                     ("learned", bh.FORM_LEARNED, bh.SCHED_MODEL, 0),
+                                     ("learned_split", bh.FORM_LEARNED, bh.SCHED_MODEL, 4)):
         say("schedule", name)
         with bh.Engine(form=form, theta=1.05) as eng:
             eng.set_schedule(sched)
+            eng.set_schedule_split(split)
             eng.model_configure(train_steps=4, iters_per_step=8, batch=1024, lr=3e-3)
             eng.set_bodies(pos, vel, m)
             eng.step(7, HOUR)   # spans training (4 steps) and model use (3 steps)
@@ -75,7 +77,7 @@ def test_learned_forces_bit_identical_to_mixed(bh):
             if form == bh.FORM_LEARNED:
                 ready, samples, updates, loss = eng.model_status()
                 assert ready and samples > 0 and updates == 32
-    for name in ("mixed_prev", "learned"):
+    for name in ("mixed_prev", "learned", "learned_split"):
         for k in ("pos", "vel", "acc"):
             assert np.array_equal(bits(res[name][k]), bits(res["mixed"][k])), (name, k)
         assert np.array_equal(res[name]["ids"], res["mixed"]["ids"])
