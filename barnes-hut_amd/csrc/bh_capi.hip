@@ -250,6 +250,9 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
                            sched == BH_SCHED_KEY_ORDER ? 0 : (split ? 2 : 1), n_chunks, order_max,
                            c->heavy_items};
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
+        // heavy bodies after the walk: run beside the tree build on a second
+        // stream they slowed the latency-bound tree kernels more than they hid
+        bh::launch_heavy_direct(c->cur, tb, c->mb, mp, s);
         if (training) {
             // training phase: the walk's list lengths become samples, then
             // iters_per_step Adam updates on mini-batches of all samples so far
@@ -497,8 +500,8 @@ int bh_create(bh_ctx** out, const bh_config* cfg)
     c->cfg = c0;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        delete c;
-        return fail(nullptr, BH_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        bh_destroy(c);
+        return fail(nullptr, BH_ERR_HIP, "stream/event creation: %s", hipGetErrorString(e));
     }
     for (auto& ev : c->ev) {
         e = hipEventCreate(&ev);

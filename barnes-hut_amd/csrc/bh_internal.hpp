@@ -240,9 +240,12 @@ struct GroupParams {
 // Walk groups and the children-contiguous node array (after the tree build).
 void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, double G,
                           hipStream_t s);
-// Persistent mixed walk over the groups; returns the number of worker waves.
+// Persistent mixed walk over the groups (light bodies: accelerations + kick).
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
                        const MixedParams& p, unsigned blocks, hipStream_t s);
+// FP64 direct sums of the heavy bodies of every system (accelerations + kick).
+void launch_heavy_direct(const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, const MixedParams& p,
+                         hipStream_t s);
 unsigned mixed_walk_blocks(int device);
 // Heaviest-first group order into mb.order (source 0: previous step's list
 // lengths; 1: mb.gcost from the cost model).

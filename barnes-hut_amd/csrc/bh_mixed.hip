@@ -939,11 +939,15 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
 {
     if (n == 0 || blocks == 0) return;
     k_walk_mixed<<<blocks, kNT, 0, s>>>(b, n, tb, mb, p);
-    if (p.heavy_items > 0) {
-        const unsigned hb = std::min<unsigned>((p.heavy_items + kWaves - 1) / kWaves, 4096u);
-        k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
-        k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kWaves - 1) / kWaves, kNT, 0, s>>>(b, tb, mb, p);
-    }
+}
+
+void launch_heavy_direct(const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, const MixedParams& p,
+                         hipStream_t s)
+{
+    if (p.heavy_items == 0) return;
+    const unsigned hb = std::min<unsigned>((p.heavy_items + kWaves - 1) / kWaves, 4096u);
+    k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
+    k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kWaves - 1) / kWaves, kNT, 0, s>>>(b, tb, mb, p);
 }
 
 }  // namespace bh
