@@ -74,6 +74,7 @@ struct bh_ctx {
     bool have_tree = false;
     bool have_counts = false;
     bool have_groups = false;
+    bool prev_counts = false;         // the bodies carry list lengths of an earlier mixed walk
     hipEvent_t ev[5] = {};
     // per-step phase events of bh_step: ring of kRing sets of 5
     std::vector<hipEvent_t> ring;
@@ -255,13 +256,18 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         bh::launch_heavy_direct(c->cur, tb, c->mb, mp, s);
         if (training) {
             // training phase: the walk's list lengths become samples, then
-            // iters_per_step Adam updates on mini-batches of all samples so far
-            bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
-            for (int k = 0; k < c->iters_per_step; ++k)
-                bh::launch_model_train(c->md, nullptr, nullptr, 0, c->batch, 1, static_cast<uint32_t>(c->updates++),
-                                       c->lr, true, nullptr, nullptr, s);
+            // iters_per_step Adam updates on mini-batches of all samples so far.
+            // The first walk of a body set has no previous list lengths
+            // (feature 5 is 0 for every group): its groups are no samples.
+            if (c->prev_counts) {
+                bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
+                for (int k = 0; k < c->iters_per_step; ++k)
+                    bh::launch_model_train(c->md, nullptr, nullptr, 0, c->batch, 1,
+                                           static_cast<uint32_t>(c->updates++), c->lr, true, nullptr, nullptr, s);
+            }
             if (++c->trained_steps >= c->train_steps) c->model_ready = true;
         }
+        c->prev_counts = true;
         MARK(4);
         HIP_TRY(c, hipGetLastError());
         if (ev) c->last_ev = ev;
@@ -557,6 +563,7 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
     c->have_tree = false;
     c->have_counts = false;
     c->have_groups = false;
+    c->prev_counts = false;
     if (n == 0) {
         c->n_sys = 1;
         c->tb.n_sys = 1;
@@ -1119,7 +1126,10 @@ int bh_model_set_params(bh_ctx* c, const float* params)
 int bh_model_reset(bh_ctx* c, uint64_t seed)
 {
     if (check_ctx(c)) return BH_ERR_ARG;
-    // He-uniform weights for the ReLU layers, a small output layer, zero biases
+    // He-uniform weights for the ReLU layers, a zero output layer and zero
+    // biases: the untrained network returns its fallback (feature 5, the
+    // previous step's list length) exactly, and training moves it only where
+    // the loss falls
     std::vector<float> w(bh::kModelParams, 0.f);
     uint64_t z = seed;
     auto uni = [&]() {
@@ -1133,7 +1143,6 @@ int bh_model_reset(bh_ctx* c, uint64_t seed)
     const int F = bh::kModelFeatures, H = 32;
     for (int k = 0; k < H * F; ++k) w[k] = static_cast<float>(uni() * std::sqrt(6.0 / F));
     for (int k = 0; k < H * H; ++k) w[H * F + H + k] = static_cast<float>(uni() * std::sqrt(6.0 / H));
-    for (int k = 0; k < H; ++k) w[H * F + H + H * H + H + k] = static_cast<float>(uni() * std::sqrt(1.0 / H));
     int rc = bh_model_set_params(c, w.data());
     if (rc) return rc;
     HIP_TRY(c, hipMemsetAsync(c->md.n_seen, 0, 8, c->stream));
