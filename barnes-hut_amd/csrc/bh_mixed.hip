@@ -373,8 +373,12 @@ __device__ inline void interact(const float4 q, float ox, float oy, float oz, fl
     const float dy = q.y - oy;
     const float dz = q.z - oz;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
-    const float inv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
-    const float s = q.w * (inv * inv * inv);
+    // The self entry (the lane's own body: dx = dy = dz = 0 exactly) has
+    // r2 = 0 and rsq = +inf; clamping s to 1e30 -- far above any physical
+    // G m / r^3 -- keeps it finite, so it adds 0 * 1e30 = +0, and it costs one
+    // instruction where a compare and a select cost two.
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float s = fminf(q.w * (inv * inv * inv), 1e30f);
     bx = fmaf(s, dx, bx);
     by = fmaf(s, dy, by);
     bz = fmaf(s, dz, bz);
