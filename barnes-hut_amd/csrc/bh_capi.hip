@@ -988,7 +988,7 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
     double best_ms = 0.0, def_ms = 0.0;
     int tried = 0, rc = BH_OK;
     const uint32_t gmaxes[3] = {32, 48, 64};
-    const double efacs[3] = {1.0, 2.0, 4.0};
+    const double efacs[4] = {1.0, 2.0, 4.0, 8.0};
     const unsigned blocks[2] = {c->mixed_blocks_max, std::max(1u, c->mixed_blocks_max / 2)};
     for (unsigned bl : blocks)
         for (uint32_t gm : gmaxes)

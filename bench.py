@@ -282,7 +282,7 @@ def main():
             continue
         log(f"bench: {name}, warmup {warmup}, steps {steps}")
         eng, wall, ph, extra = run_form(bh, forms[name], pos, vel, mass, theta, steps, warmup, systems=systems,
-                                        tune=systems is not None and name != "plain")
+                                        tune=name != "plain")
         r = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3,
                  phase_ms={"kick_drift_box_keys": ph[0], "sort_gather": ph[1], "tree_groups_schedule": ph[2],
                            "walk_kick": ph[3], "total_device": ph[4]},
