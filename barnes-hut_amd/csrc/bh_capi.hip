@@ -226,7 +226,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     }
     bh::launch_build_tree(c->cur, c->n, tb, s);
     if (mixed) {
-        bh::launch_mixed_prepare(c->n, tb, c->mb, c->gp, G, s);
+        bh::launch_mixed_prepare(c->cur, c->n, tb, c->mb, c->gp, G, s);
         int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;
         const bool learned = form == BH_FORM_LEARNED;
         const bool training = learned && c->trained_steps < c->train_steps;

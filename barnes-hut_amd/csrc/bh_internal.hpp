@@ -75,9 +75,18 @@ struct StepScalars {
     uint32_t block_counter;  // last-block-done counter, left at 0
     uint32_t error;          // kErr* bits raised by bounds guards
     uint32_t n_groups;       // walk groups of the mixed forms
-    uint32_t group_counter;  // dynamic work queue of the mixed walk
+    uint32_t group_counter;  // dynamic work queue of the mixed walk (cost order, split schedule)
     uint32_t cost_total;     // sum of the group weights of the split schedule
+    uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
 };
+
+// In key order the mixed walk deals its groups out in runs of kWalkRun
+// consecutive groups, run r to part r mod kWalkParts, one part per XCD
+// (workgroup b runs on XCD b mod 8): neighbouring groups share their
+// near-field nodes, and a run keeps them in one XCD's L2.  A part whose runs
+// are done takes the runs of the following parts.
+constexpr uint32_t kWalkParts = 8;
+constexpr uint32_t kWalkRun = 32;
 
 constexpr uint32_t kErrWalkIndex = 1u;   // a walk index fell outside [0, M+B)
 constexpr uint32_t kErrStack = 2u;       // a mixed-walk stack would have overflowed
@@ -185,7 +194,8 @@ void launch_walk_plain(const Bodies& b, size_t n, const TreeBuffers& tb, const W
 // opening a node pushes one (first, count) range.  Masses and centres are
 // those of the light bodies; the heavy bodies are summed directly in FP64.
 struct MNode {
-    double x, y, z;   // centre of mass of the light bodies (FP64)
+    double x, y, z;   // centre of mass of the light bodies (FP64); a one-body
+                      // leaf holds the body's own position
     double size2;     // opening size^2: squared distance from the centre of mass
                       // to the farthest corner of the light bodies' box
     float gm;         // G * light mass
@@ -214,8 +224,8 @@ struct MixedBuffers {
     uint32_t* chunk;    // split schedule: first group of each chunk (n_chunks + 1, <= 65536)
     float* gcost;       // N: estimated cost of each group
     double* hpart;      // heavy direct sums: 3 partials per work item (system, heavy rank, chunk)
-    uint32_t* sched_hist;    // 64 cost buckets (left at zero)
-    uint32_t* sched_cursor;  // 64
+    uint32_t* sched_hist;    // 64 cost buckets per group part (left at zero)
+    uint32_t* sched_cursor;  // 64 per part
 };
 
 struct MixedParams {
@@ -238,8 +248,8 @@ struct GroupParams {
 };
 
 // Walk groups and the children-contiguous node array (after the tree build).
-void launch_mixed_prepare(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, double G,
-                          hipStream_t s);
+void launch_mixed_prepare(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
+                          double G, hipStream_t s);
 // Persistent mixed walk over the groups (light bodies: accelerations + kick).
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
                        const MixedParams& p, unsigned blocks, hipStream_t s);

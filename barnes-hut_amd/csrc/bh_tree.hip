@@ -214,6 +214,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
         sc->group_counter = 0;
         sc->block_counter = 0;
     }
+    if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------- keys
