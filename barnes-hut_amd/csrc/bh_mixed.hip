@@ -338,6 +338,18 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
+// lane k's value (k wave-uniform)
+__device__ inline uint32_t lane_u32(uint32_t v, uint32_t k)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(k)));
+}
+__device__ inline double lane_f64(double v, uint32_t k)
+{
+    const uint64_t u = static_cast<uint64_t>(__double_as_longlong(v));
+    const uint64_t lo = lane_u32(static_cast<uint32_t>(u), k), hi = lane_u32(static_cast<uint32_t>(u >> 32), k);
+    return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+}
+
 // number of set lanes of mask below this lane
 __device__ inline uint32_t rank_below(uint64_t mask)
 {
@@ -432,6 +444,17 @@ __device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, 
 }
 
 constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
+
+// Entry s of a round (ent) and, for a node entry, its node; a zero node (no
+// mass: never used) for body entries and past the round's end.
+__device__ inline MNode fetch_node(const MixedBuffers& mb, const uint32_t* nbuf, uint32_t s, uint32_t total,
+                                   uint32_t& ent)
+{
+    MNode nd{};
+    ent = s < total ? nbuf[s] : 0u;
+    if (s < total && !(ent & 0x80000000u)) nd = mb.node[ent & 0x7fffffffu];
+    return nd;
+}
 
 // 135 VGPRs: 3 waves per SIMD.  Capping at 128 for 4 waves measured slower on
 // the 300k and ensemble walks (the list loop needs its registers).
@@ -529,14 +552,25 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         double hax = 0.0, hay = 0.0, haz = 0.0;
         const SysScalars* sy = tb.sys + gsys;
         const uint32_t nh = sy->n_heavy;
+        // lane k loads heavy body k (one load latency for all of them); the
+        // loop reads them back as wave constants
+        uint32_t hj = kNone;
+        double hpx = 0.0, hpy = 0.0, hpz = 0.0, hpm = 0.0;
+        if (static_cast<uint32_t>(lane) < nh) {
+            hj = sy->heavy_slot[lane];
+            hpx = b.px[hj];
+            hpy = b.py[hj];
+            hpz = b.pz[hj];
+            hpm = b.m[hj];
+        }
         for (uint32_t k = 0; k < nh; ++k) {
-            const uint32_t j = sy->heavy_slot[k];
+            const uint32_t j = lane_u32(hj, k);
             if (j == i) continue;
-            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+            const double dx = lane_f64(hpx, k) - x, dy = lane_f64(hpy, k) - y, dz = lane_f64(hpz, k) - z;
             const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
             if (!(r2 > 0.0)) continue;
             const double invR = 1.0 / sqrt(r2);
-            const double s = p.G * b.m[j] * (invR * invR * invR);
+            const double s = p.G * lane_f64(hpm, k) * (invR * invR * invR);
             hax += s * dx;
             hay += s * dy;
             haz += s * dz;
@@ -566,10 +600,16 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
             }
             __builtin_amdgcn_wave_barrier();
             n_visit += total;
+            // the nodes of the next 64 entries are loaded before this chunk is
+            // tested, so their latency overlaps its work
+            uint32_t ent_n = 0u;
+            MNode nd_n = fetch_node(mb, nbuf, static_cast<uint32_t>(lane), total, ent_n);
             for (uint32_t base = 0; base < total; base += 64u) {
                 const uint32_t s = base + static_cast<uint32_t>(lane);
                 const bool act = s < total;
-                const uint32_t ent = act ? nbuf[s] : 0u;
+                const uint32_t ent = ent_n;
+                const MNode nd = nd_n;
+                nd_n = fetch_node(mb, nbuf, s + 64u, total, ent_n);
                 bool app = false;
                 float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
                 uint32_t npush = 0, pfirst = 0, pcount = 0;
@@ -584,7 +624,6 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                                             static_cast<float>(b.pz[id] - cz), static_cast<float>(p.G * mb_));
                         }
                     } else {
-                        const MNode nd = mb.node[id];
                         if (nd.gm != 0.f) {
                             const uint32_t nc = nd.info & ~kMLeaf;
                             if (nd.info & kMLeaf) {
