@@ -31,6 +31,7 @@ ERR_ARG, ERR_NOMEM, ERR_HIP, ERR_STATE, ERR_IO, ERR_UNSUPPORTED, ERR_OVERFLOW = 
 FC_TREE, FC_LET, FC_LET_DIRECT = 0, 1, 2
 FORM_PLAIN, FORM_MIXED, FORM_LEARNED = 0, 1, 2
 SCHED_KEY_ORDER, SCHED_PREV_COST, SCHED_MODEL = 0, 1, 2
+SORT_NEAR, SORT_FULL = 0, 1
 GEN_BELT, GEN_KUIPER, GEN_TROJAN = 0, 1, 2
 G = 1.48812e-34
 MAX_DEPTH = 21
@@ -105,6 +106,8 @@ SIGNATURES = [
     ("bh_autotune", C.c_int, [_P, C.c_int, C.c_int, _P]),
     ("bh_set_relative_opening", C.c_int, [_P, _D]),
     ("bh_set_schedule_split", C.c_int, [_P, C.c_int]),
+    ("bh_set_sort_mode", C.c_int, [_P, C.c_int]),
+    ("bh_get_sort_stats", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_I64)]),
     ("bh_get_relative_opening", C.c_int, [_P, C.POINTER(_D)]),
 ]
 
@@ -314,6 +317,18 @@ class Engine:
 
     def set_schedule_split(self, chunks_per_worker: int):
         self._ck(self.lib.bh_set_schedule_split(self.h, chunks_per_worker))
+
+    def set_sort_mode(self, mode: int):
+        """SORT_NEAR (default): sort only the keys out of order since the
+        previous step and merge; SORT_FULL: full radix sort every step."""
+        self._ck(self.lib.bh_set_sort_mode(self.h, mode))
+
+    def sort_stats(self):
+        """(took_near_path, keys_radix_sorted) of the last key sort."""
+        near = C.c_int(0)
+        k = _I64(0)
+        self._ck(self.lib.bh_get_sort_stats(self.h, C.byref(near), C.byref(k)))
+        return bool(near.value), int(k.value)
 
     # ------------------------------------------------ graphs, ensembles, tuning
     def set_graph(self, on: bool):

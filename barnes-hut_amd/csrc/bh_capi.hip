@@ -57,6 +57,7 @@ struct bh_ctx {
     int64_t graph_replays = 0;
     int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
     int split = 0;                    // chunks per worker of the split schedule (0: heaviest first)
+    bool sort_near = true;            // key sort: near-sorted front end (false: full radix sort)
     bool model_ready = false;         // the LEARNED cost model is trained
     // cost model (allocated once per context, independent of N)
     ModelBuffers md{};
@@ -162,7 +163,7 @@ int reserve(bh_ctx* c, size_t n)
     A(c->cur.ax, N); A(c->cur.ay, N); A(c->cur.az, N);
     c->alt.ax = c->cur.ax; c->alt.ay = c->cur.ay; c->alt.az = c->cur.az;
     TreeBuffers& t = c->tb;
-    A(t.key, N); A(t.key_alt, N); A(t.perm, N); A(t.perm_alt, N);
+    A(t.key, N); A(t.key_alt, N); A(t.perm, N); A(t.perm_alt, N); A(t.key_alt2, N); A(t.perm_alt2, N);
     A(t.flag, N); A(t.leaf_of, N); A(t.self_w, N);
     A(t.ukey, N); A(t.leaf_first, N + 1); A(t.delta, N); A(t.rep, N); A(t.is_rep, N);
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
@@ -213,8 +214,12 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb, c->partials, s);
     bh::launch_keys(c->cur, c->n, c->tb, s);
     MARK(1);
-    bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63,
-                         c->n_sys > 1 ? c->tb.bsys : nullptr, c->tb.sort_tmp, c->cap, s);
+    if (c->sort_near)
+        bh::sort_near_sorted(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->tb.key_alt2, c->tb.perm_alt2,
+                             c->n, 63, c->n_sys > 1 ? c->tb.bsys : nullptr, c->tb.sort_tmp, c->cap, s);
+    else
+        bh::radix_sort_pairs(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->n, 63,
+                             c->n_sys > 1 ? c->tb.bsys : nullptr, c->tb.sort_tmp, c->cap, s);
     bh::launch_permute(c->cur, c->alt, c->tb, c->n, s);
     std::swap(c->cur, c->alt);
     c->parity ^= 1;
@@ -426,7 +431,7 @@ std::vector<double> graph_key(const bh_ctx* c, double dt)
     return {static_cast<double>(c->cfg.form), c->cfg.theta, c->cfg.G, c->cfg.soft2, dt, static_cast<double>(c->n),
             static_cast<double>(c->n_sys), static_cast<double>(c->sched), c->model_ready ? 1.0 : 0.0,
             static_cast<double>(c->gp.gmax), c->gp.efac, static_cast<double>(c->gp.occ), c->mac_alpha,
-            static_cast<double>(c->split),
+            static_cast<double>(c->split), c->sort_near ? 1.0 : 0.0,
             static_cast<double>(c->mixed_blocks), static_cast<double>(c->cap)};
 }
 
@@ -1028,6 +1033,26 @@ int bh_set_schedule_split(bh_ctx* c, int chunks_per_worker)
     if (check_ctx(c)) return BH_ERR_ARG;
     if (chunks_per_worker < 0 || chunks_per_worker > 64) return fail(c, BH_ERR_ARG, "0..64 chunks per worker");
     c->split = chunks_per_worker;
+    return BH_OK;
+}
+
+int bh_set_sort_mode(bh_ctx* c, int mode)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (mode != BH_SORT_NEAR && mode != BH_SORT_FULL) return fail(c, BH_ERR_ARG, "unknown sort mode %d", mode);
+    c->sort_near = mode == BH_SORT_NEAR;
+    return BH_OK;
+}
+
+int bh_get_sort_stats(bh_ctx* c, int* near_path, int64_t* sorted_keys)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (!c->tb.sort_tmp) return fail(c, BH_ERR_STATE, "no bodies");
+    uint32_t ctl[4] = {0, 0, 0, 0};
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(ctl, bh::sort_near_ctl(c->tb.sort_tmp, c->cap), sizeof(ctl), hipMemcpyDeviceToHost));
+    if (near_path) *near_path = c->sort_near ? static_cast<int>(ctl[0]) : 0;
+    if (sorted_keys) *sorted_keys = c->sort_near ? (ctl[0] ? ctl[3] : static_cast<int64_t>(c->n)) : static_cast<int64_t>(c->n);
     return BH_OK;
 }
 

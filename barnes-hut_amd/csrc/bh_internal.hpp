@@ -114,6 +114,15 @@ size_t sort_tmp_words(size_t cap);
 // orders by sysd[payload], i.e. by (system, key).
 void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t n, int bits,
                       const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s);
+// The same result for an input that is nearly sorted already (the previous
+// step's order): only the keys out of order with their neighbours are radix
+// sorted, then merged with the others; falls back to sorting every key when
+// the rest is not in order.  Buffers (k1, v1) and (k2, v2) hold n entries.
+void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t* k2, uint32_t* v2, size_t n,
+                      int bits, const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s);
+// device address of its control words: [0] 1 if the last call took the
+// near-sorted path, [3] the keys it radix sorted there
+uint32_t* sort_near_ctl(uint32_t* sort_tmp, size_t cap);
 uint32_t* sort_error_word(uint32_t* sort_tmp);
 
 // ----------------------------------------------------------------- step
@@ -130,6 +139,8 @@ struct TreeBuffers {
     uint64_t* key_alt;
     uint32_t* perm;       // sort payload (gather index)
     uint32_t* perm_alt;
+    uint64_t* key_alt2;   // second scratch pair of the near-sorted sort
+    uint32_t* perm_alt2;
     uint32_t* flag;       // first-of-key flags, then leaf index (exclusive scan)
     uint32_t* leaf_of;    // leaf id of each body
     uint32_t* self_w;     // walk index of each body's leaf

@@ -7,9 +7,9 @@
 // publishes its own inclusive prefix.  A status word is 64 bits -- tag
 // ((epoch + 1) << 1 | inclusive) over the 32-bit value -- written and read
 // with relaxed device-scope atomics, so the value travels with its flag and
-// no cache flush is needed.  The epoch advances once per launch (the last
-// block to finish bumps it and resets the counters), so words left by earlier
-// launches never match.  A tile only waits for tiles with smaller ids, which
+// no cache flush is needed.  The counter packs (epoch, next id); the epoch
+// advances once per launch (the block that takes the last id starts the next
+// epoch), so words left by earlier launches never match.  A tile only waits for tiles with smaller ids, which
 // were handed to blocks that are already running and publish their
 // aggregates before they wait for anything; each wait is still bounded and
 // sets the chain's error word (read by the host after the step) instead of
@@ -50,7 +50,7 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 // look-back state at the head of a tmp buffer: counters, then status words
 struct Chain {
-    uint32_t* ctr;       // [0] next tile id, [1] finished blocks, [2] epoch, [3] error
+    uint32_t* ctr;       // [0, 1] one 64-bit word: epoch << 32 | next tile id; [3] error
     uint64_t* status;    // width words per tile
 };
 constexpr size_t kChainHeadWords = 4;
@@ -113,33 +113,27 @@ __device__ inline uint32_t look_back(const Chain& c, uint32_t tile, uint32_t wid
     return excl;
 }
 
-// Thread 0 takes the next tile id; every thread gets (id, epoch).  An id
-// beyond the grid (counters not reset, which a completed launch always does)
-// raises the error word; the caller then skips its tile.
-__device__ inline bool chain_enter(const Chain& c, uint32_t* s_id, uint32_t* s_epoch)
+// Each of the launch's `active` blocks (blockIdx.x < active; the others
+// never touch the chain) takes (epoch, tile id) with one atomic on the
+// packed counter.  The block that takes the last id -- every other id is
+// taken by then -- starts the next epoch at id 0 for the next launch.  An id
+// at or past `active` (a counter left by an aborted launch) raises the error
+// word; the caller then skips its tile.
+__device__ inline bool chain_enter(const Chain& c, uint32_t active, uint32_t* s_id, uint32_t* s_epoch)
 {
     if (threadIdx.x == 0) {
-        *s_id = atomicAdd(c.ctr, 1u);
-        *s_epoch = __hip_atomic_load(c.ctr + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (*s_id >= gridDim.x) atomicOr(c.ctr + 3, 2u);
+        uint64_t* ctr64 = reinterpret_cast<uint64_t*>(c.ctr);
+        const uint64_t v = atomicAdd(reinterpret_cast<unsigned long long*>(ctr64), 1ull);
+        const uint32_t id = static_cast<uint32_t>(v), epoch = static_cast<uint32_t>(v >> 32);
+        if (id == active - 1u)
+            __hip_atomic_store(ctr64, static_cast<uint64_t>(epoch + 1u) << 32, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (id >= active) atomicOr(c.ctr + 3, 2u);
+        *s_id = id;
+        *s_epoch = epoch;
     }
     __syncthreads();
-    return *s_id < gridDim.x;
-}
-
-// After the block's last status write: the last block to finish resets the
-// counters and advances the epoch for the next launch.
-__device__ inline void chain_leave(const Chain& c, uint32_t epoch)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t prev = atomicAdd(c.ctr + 1, 1u);
-        if (prev == gridDim.x - 1) {
-            __hip_atomic_store(c.ctr + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(c.ctr + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(c.ctr + 2, epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    return *s_id < active;
 }
 
 __device__ inline uint32_t wave_inclusive_scan(uint32_t v)
@@ -273,25 +267,54 @@ __host__ __device__ inline SortTmp sort_tmp_at(uint32_t* tmp)
 // Digit counts of every pass: per-block LDS histograms added into the global
 // counts; the last block turns them into each pass's exclusive digit offsets
 // (base) and clears the counts and its counter for the next sort.
+// With ctl (the near-sorted front end's control words) the count is ctl[1]
+// and, in near mode (ctl[0] == 1), the input is (key_near, val_near).
 template <int kSortIPT>
-__global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restrict__ key, size_t n, int key_passes,
-                                                       const uint8_t* __restrict__ sysd, uint32_t* tmp)
+__global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val,
+                                                       size_t n, int key_passes, const uint8_t* __restrict__ sysd,
+                                                       uint32_t* tmp, const uint64_t* __restrict__ key_near,
+                                                       const uint32_t* __restrict__ val_near, const uint32_t* ctl)
 {
     __shared__ uint32_t h[kMaxPasses][kDigits];
+    __shared__ uint32_t lds[kWaves + 1];
     __shared__ bool last;
     const SortTmp st = sort_tmp_at(tmp);
     const int passes = key_passes + (sysd ? 1 : 0);
+    if (ctl) {
+        n = ctl[1];
+        if (ctl[0] == 1u) {
+            key = key_near;
+            val = val_near;
+        }
+    }
+    const uint32_t active = static_cast<uint32_t>((n + kNT * kSortIPT - 1) / (kNT * kSortIPT));
+    if (blockIdx.x >= active) return;   // blocks past the count
+    const int lane = threadIdx.x & 63;
     for (int k = threadIdx.x; k < kMaxPasses * kDigits; k += kNT) (&h[0][0])[k] = 0;
     __syncthreads();
     const size_t base = static_cast<size_t>(blockIdx.x) * (kNT * kSortIPT);
+    // a digit shared by the whole wavefront (the high digits of sorted
+    // keys) is added once instead of 64 times to one LDS word
+    auto count = [&](uint32_t* row, uint32_t d, bool valid, uint64_t act, int first) {
+        const uint32_t d0 = static_cast<uint32_t>(__shfl(static_cast<int>(d), first, 64));
+        const uint64_t same = __ballot(valid && d == d0);
+        if (same == act) {
+            if (lane == first) atomicAdd(&row[d0], static_cast<uint32_t>(__popcll(act)));
+        } else if (valid) {
+            atomicAdd(&row[d], 1u);
+        }
+    };
 #pragma unroll
     for (int j = 0; j < kSortIPT; ++j) {
         const size_t idx = base + static_cast<size_t>(j) * kNT + threadIdx.x;
-        if (idx < n) {
-            const uint64_t k = key[idx];
-            for (int p = 0; p < key_passes; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
-            if (sysd) atomicAdd(&h[key_passes][sysd[idx]], 1u);   // the payload is still the identity
-        }
+        const bool valid = idx < n;
+        const uint64_t act = __ballot(valid);
+        if (act == 0) continue;
+        const int first = __builtin_ctzll(act);
+        const uint64_t k = valid ? key[idx] : 0ull;
+        for (int p = 0; p < key_passes; ++p)
+            count(h[p], static_cast<uint32_t>(k >> (8 * p)) & 255u, valid, act, first);
+        if (sysd) count(h[key_passes], valid ? static_cast<uint32_t>(sysd[val[idx]]) : 0u, valid, act, first);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < passes * kDigits; k += kNT) {
@@ -302,7 +325,7 @@ __global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restric
     if (threadIdx.x == 0) {
         __threadfence();
         const uint32_t prev = atomicAdd(st.done, 1u);
-        last = (prev == gridDim.x - 1);
+        last = (prev == active - 1);
     }
     __syncthreads();
     if (!last) return;
@@ -312,13 +335,9 @@ __global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restric
         __hip_atomic_store(st.count + k, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    if (threadIdx.x < passes) {
-        uint32_t run = 0;
-        for (int d = 0; d < kDigits; ++d) {
-            const uint32_t t = h[threadIdx.x][d];
-            st.base[threadIdx.x * kDigits + d] = run;
-            run += t;
-        }
+    for (int p = 0; p < passes; ++p) {   // thread d: digit d of pass p
+        uint32_t total;
+        st.base[p * kDigits + threadIdx.x] = block_exclusive_scan(h[p][threadIdx.x], &total, lds);
     }
     if (threadIdx.x == 0) __hip_atomic_store(st.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -326,7 +345,9 @@ __global__ __launch_bounds__(kNT) void k_sort_hist_all(const uint64_t* __restric
 template <int kSortIPT>
 __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
-                                                   int pass, const uint8_t* __restrict__ sysd, uint32_t* tmp)
+                                                   int pass, const uint8_t* __restrict__ sysd, uint32_t* tmp,
+                                                   const uint64_t* __restrict__ kin_near,
+                                                   const uint32_t* __restrict__ vin_near, const uint32_t* ctl)
 {
     __shared__ uint32_t wcnt[kWaves][kDigits];
     __shared__ uint32_t boff[kDigits];
@@ -335,11 +356,18 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
     const Chain c = chain_at(st.chain);
     const int shift = 8 * pass;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = 0; k < kWaves; ++k) wcnt[k][threadIdx.x] = 0;
-    if (!chain_enter(c, &s_id, &s_epoch)) {   // includes the barrier after the clears
-        chain_leave(c, s_epoch);
-        return;
+    if (ctl) {
+        n = ctl[1];
+        if (ctl[0] == 1u && kin_near) {
+            kin = kin_near;
+            vin = vin_near;
+        }
     }
+    // blocks past the tiles of the count return at once
+    const uint32_t active = static_cast<uint32_t>((n + kNT * kSortIPT - 1) / (kNT * kSortIPT));
+    if (blockIdx.x >= active) return;
+    for (int k = 0; k < kWaves; ++k) wcnt[k][threadIdx.x] = 0;
+    if (!chain_enter(c, active, &s_id, &s_epoch)) return;   // includes the barrier after the clears
     const uint32_t tile = s_id, epoch = s_epoch;
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -405,7 +433,290 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
             vout[pos] = vv[j];
         }
     }
-    chain_leave(c, epoch);
+}
+
+// ------------------------------------------------- near-sorted front end
+// The bodies keep the previous step's sorted order, so after a drift only a
+// few percent of the new keys are out of place.  A key is kept when it is in
+// order against its kNsWin neighbours on either side (bodies of different
+// systems are always in order: a system owns a fixed index range); if the
+// kept keys, in index order, are then sorted as a whole, only the removed
+// ones go through the radix sort, and a merge of the two sorted runs gives
+// the unique stable order -- equal keys by input position -- which the full
+// radix sort would give.  If the check fails, or more than half of the keys
+// were removed, every key goes through the radix sort (full mode).
+//
+// Control words (ctl): [0] mode (1 near, 0 full), [1] count of the radix
+// sort (removed keys, or n), [2] kept keys (0 in full mode), [3] keys of the
+// sorted run the merge takes (= [1]), [5] removed keys.  (A one-workgroup LDS
+// sort of up to 12288 removed keys measured 94 us at 300k bodies -- one CU
+// doing eight passes -- against 91 us for the multi-block passes.)
+constexpr int kNsWin = 4;
+constexpr int kNsIPT = 8;
+constexpr int kNsTile = kNT * kNsIPT;   // 2048 keys per tile
+constexpr int kNsBackScan = 64;         // longest run of removed keys checked inside a tile
+
+__host__ __device__ inline size_t ns_tiles(size_t n) { return (n + kNsTile - 1) / kNsTile > 0 ? (n + kNsTile - 1) / kNsTile : 1; }
+
+struct NsTmp {
+    uint32_t* ctl;     // 8
+    uint32_t* tcnt;    // kept keys per tile
+    uint32_t* tfirst;  // index of the tile's first kept key (kNone: none)
+    uint32_t* tlast;   // index of its last kept key
+    uint32_t* tok;     // 1: the tile's kept keys are in order among themselves
+    uint32_t* toff;    // exclusive prefix of tcnt
+    uint8_t* flag;     // 1: kept
+};
+constexpr uint32_t kNone32 = 0xffffffffu;
+
+__host__ __device__ inline NsTmp ns_tmp_at(uint32_t* tmp, size_t cap)
+{
+    NsTmp t;
+    const size_t T = ns_tiles(cap);
+    t.ctl = sort_tmp_at(tmp).chain + kChainHeadWords + 2 * kDigits * sort_tiles(cap);
+    t.tcnt = t.ctl + 8;
+    t.tfirst = t.tcnt + T;
+    t.tlast = t.tfirst + T;
+    t.tok = t.tlast + T;
+    t.toff = t.tok + T;
+    t.flag = reinterpret_cast<uint8_t*>(t.toff + T);
+    return t;
+}
+__host__ __device__ inline size_t ns_tmp_words(size_t cap) { return 8 + 5 * ns_tiles(cap) + (cap + 3) / 4; }
+
+// a at index i before b at index j > i is in order
+__device__ inline bool ns_in_order(uint64_t ka, uint8_t sa, uint64_t kb, uint8_t sb) { return sa != sb || ka <= kb; }
+
+// Kept flags of one tile of 2048 keys (8 per thread, thread-strided) and the
+// tile's kept count, first and last kept index and inner order.
+__global__ __launch_bounds__(kNT) void k_ns_mark(const uint64_t* __restrict__ key, size_t n,
+                                                 const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
+{
+    __shared__ uint64_t sk[kNsTile + 2 * kNsWin];
+    __shared__ uint8_t ss[kNsTile + 2 * kNsWin];
+    __shared__ uint8_t sf[kNsTile];
+    __shared__ uint32_t r_cnt[kWaves], r_first[kWaves], r_last[kWaves], r_ok[kWaves];
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kNsTile;
+    for (int e = threadIdx.x; e < kNsTile + 2 * kNsWin; e += kNT) {
+        const int64_t i = t0 + e - kNsWin;
+        const bool in = i >= 0 && i < static_cast<int64_t>(n);
+        sk[e] = in ? key[i] : 0ull;
+        ss[e] = (in && sysd) ? sysd[i] : 0;
+    }
+    __syncthreads();
+    uint32_t cnt = 0, first = kNone32, last = 0;
+#pragma unroll
+    for (int j = 0; j < kNsIPT; ++j) {
+        const int e = j * kNT + threadIdx.x;   // tile-relative index
+        const int64_t i = t0 + e;
+        bool good = false;
+        if (i < static_cast<int64_t>(n)) {
+            good = true;
+            const int x = e + kNsWin;
+#pragma unroll
+            for (int d = 1; d <= kNsWin; ++d) {
+                if (i - d >= 0) good = good && ns_in_order(sk[x - d], ss[x - d], sk[x], ss[x]);
+                if (i + d < static_cast<int64_t>(n)) good = good && ns_in_order(sk[x], ss[x], sk[x + d], ss[x + d]);
+            }
+            nt.flag[i] = good ? 1 : 0;
+            if (good) {
+                ++cnt;
+                first = min(first, static_cast<uint32_t>(i));
+                last = max(last, static_cast<uint32_t>(i));
+            }
+        }
+        sf[e] = good ? 1 : 0;
+    }
+    __syncthreads();
+    // a kept key whose kept predecessor in the tile lies beyond the window
+    // is compared with it directly (runs longer than kNsBackScan fail)
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < kNsIPT; ++j) {
+        const int e = j * kNT + threadIdx.x;
+        if (!sf[e]) continue;
+        int q = e - 1, steps = 0;
+        while (q >= 0 && !sf[q] && steps < kNsBackScan) {
+            --q;
+            ++steps;
+        }
+        if (q >= 0 && !sf[q]) ok = false;   // gave up
+        else if (q >= 0 && e - q > kNsWin)
+            ok = ok && ns_in_order(sk[q + kNsWin], ss[q + kNsWin], sk[e + kNsWin], ss[e + kNsWin]);
+    }
+    uint32_t okw = ok ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += static_cast<uint32_t>(__shfl_xor(static_cast<int>(cnt), o, 64));
+        first = min(first, static_cast<uint32_t>(__shfl_xor(static_cast<int>(first), o, 64)));
+        last = max(last, static_cast<uint32_t>(__shfl_xor(static_cast<int>(last), o, 64)));
+        okw &= static_cast<uint32_t>(__shfl_xor(static_cast<int>(okw), o, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        r_cnt[w] = cnt;
+        r_first[w] = first;
+        r_last[w] = last;
+        r_ok[w] = okw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, f = kNone32, l = 0, k = 1;
+        for (int q = 0; q < kWaves; ++q) {
+            c += r_cnt[q];
+            f = min(f, r_first[q]);
+            l = max(l, r_last[q]);
+            k &= r_ok[q];
+        }
+        nt.tcnt[blockIdx.x] = c;
+        nt.tfirst[blockIdx.x] = f;
+        nt.tlast[blockIdx.x] = l;
+        nt.tok[blockIdx.x] = k;
+    }
+}
+
+// One block: tile offsets, the order across tiles, the mode.
+__global__ __launch_bounds__(kNT) void k_ns_plan(const uint64_t* __restrict__ key, size_t n,
+                                                 const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
+{
+    __shared__ uint32_t lds[kWaves + 1];
+    __shared__ uint32_t s_last[kNT];
+    __shared__ uint32_t s_bad;
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t T = static_cast<uint32_t>(ns_tiles(n));
+    if (threadIdx.x == 0) s_bad = 0;
+    uint32_t carry = 0, carry_last = kNone32;   // kept so far; last kept index of the tiles so far
+    for (uint32_t c0 = 0; c0 < T; c0 += kNT) {
+        const uint32_t t = c0 + threadIdx.x;
+        const bool in = t < T;
+        const uint32_t cnt = in ? nt.tcnt[t] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(cnt, &total, lds);
+        if (in) nt.toff[t] = carry + ex;
+        // the nearest kept key before this tile: the largest last-kept index
+        // of the earlier tiles (indices grow with the tile)
+        s_last[threadIdx.x] = (in && cnt) ? nt.tlast[t] : kNone32;
+        __syncthreads();
+        uint32_t prev = carry_last;
+        for (int k = static_cast<int>(threadIdx.x) - 1; k >= 0; --k)
+            if (s_last[k] != kNone32) {
+                prev = s_last[k];
+                break;
+            }
+        bool bad = in && !nt.tok[t];
+        if (in && cnt && prev != kNone32) {
+            const uint32_t f = nt.tfirst[t];
+            const uint8_t sa = sysd ? sysd[prev] : 0, sb = sysd ? sysd[f] : 0;
+            if (!ns_in_order(key[prev], sa, key[f], sb)) bad = true;
+        }
+        if (bad) atomicOr(&s_bad, 1u);
+        for (int k = kNT - 1; k >= 0; --k)
+            if (s_last[k] != kNone32) {
+                carry_last = s_last[k];
+                break;
+            }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t kept = carry, removed = static_cast<uint32_t>(n) - kept;
+        const bool near = !s_bad && removed <= static_cast<uint32_t>(n / 2);
+        nt.ctl[0] = near ? 1u : 0u;
+        nt.ctl[1] = near ? removed : static_cast<uint32_t>(n);
+        nt.ctl[2] = near ? kept : 0u;
+        nt.ctl[3] = nt.ctl[1];
+        nt.ctl[5] = removed;
+    }
+}
+
+// Near mode: kept keys (in index order) to (k1, v1) + R, removed ones to
+// (k2, v2), both with their payloads.
+__global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__ k0, const uint32_t* __restrict__ v0,
+                                                    size_t n, uint64_t* __restrict__ k1, uint32_t* __restrict__ v1,
+                                                    uint64_t* __restrict__ k2, uint32_t* __restrict__ v2,
+                                                    uint32_t* tmp, size_t cap)
+{
+    __shared__ uint32_t lds[kWaves + 1];
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    if (nt.ctl[0] != 1u) return;
+    const uint32_t R = nt.ctl[3];
+    const size_t base = static_cast<size_t>(blockIdx.x) * kNsTile + static_cast<size_t>(threadIdx.x) * kNsIPT;
+    uint32_t f[kNsIPT], cnt = 0;
+#pragma unroll
+    for (int j = 0; j < kNsIPT; ++j) {
+        f[j] = (base + j < n) ? nt.flag[base + j] : 0u;
+        cnt += f[j];
+    }
+    uint32_t total;
+    uint32_t kept_before = nt.toff[blockIdx.x] + block_exclusive_scan(cnt, &total, lds);
+#pragma unroll
+    for (int j = 0; j < kNsIPT; ++j) {
+        const size_t i = base + j;
+        if (i >= n) break;
+        if (f[j]) {
+            k1[R + kept_before] = k0[i];
+            v1[R + kept_before] = v0[i];
+            ++kept_before;
+        } else {
+            const size_t r = i - kept_before;
+            k2[r] = k0[i];
+            v2[r] = v0[i];
+        }
+    }
+}
+
+// (system, key, payload) order; the payload is the input position
+__device__ inline bool ns_less(uint8_t sa, uint64_t ka, uint32_t va, uint8_t sb, uint64_t kb, uint32_t vb)
+{
+    if (sa != sb) return sa < sb;
+    if (ka != kb) return ka < kb;
+    return va < vb;
+}
+
+// number of entries of the sorted run (ks, vs)[0, len) ordered before (s, k, v)
+__device__ inline uint32_t ns_rank(const uint64_t* ks, const uint32_t* vs, uint32_t len, const uint8_t* sysd, uint8_t s,
+                                   uint64_t k, uint32_t v)
+{
+    uint32_t lo = 0;
+    while (len > 0) {
+        const uint32_t half = len >> 1;
+        const uint32_t m = lo + half;
+        const uint32_t vm = vs[m];
+        if (ns_less(sysd ? sysd[vm] : 0, ks[m], vm, s, k, v)) {
+            lo = m + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// Merge of the kept run ((k1, v1) + R, K keys) and the sorted removed run
+// (fk, fv: R keys) into (k0, v0).  Full mode: K = 0, a copy.
+__global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
+                                                  const uint64_t* __restrict__ k1, const uint32_t* __restrict__ v1,
+                                                  const uint64_t* __restrict__ fk, const uint32_t* __restrict__ fv,
+                                                  const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= n) return;
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t R = nt.ctl[3], K = nt.ctl[2];
+    uint64_t k;
+    uint32_t v, pos;
+    if (i < K) {
+        k = k1[R + i];
+        v = v1[R + i];
+        pos = static_cast<uint32_t>(i) + ns_rank(fk, fv, R, sysd, sysd ? sysd[v] : 0, k, v);
+    } else {
+        const uint32_t u = static_cast<uint32_t>(i) - K;
+        k = fk[u];
+        v = fv[u];
+        pos = u + ns_rank(k1 + R, v1 + R, K, sysd, sysd ? sysd[v] : 0, k, v);
+    }
+    k0[pos] = k;
+    v0[pos] = v;
 }
 
 }  // namespace
@@ -424,7 +735,42 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size
 
 size_t sort_tmp_words(size_t cap)
 {
-    return 4 + 2 * kMaxPasses * kDigits + kChainHeadWords + 2 * kDigits * sort_tiles(cap);
+    return 4 + 2 * kMaxPasses * kDigits + kChainHeadWords + 2 * kDigits * sort_tiles(cap) + ns_tmp_words(cap);
+}
+
+uint32_t* sort_near_ctl(uint32_t* sort_tmp, size_t cap) { return ns_tmp_at(sort_tmp, cap).ctl; }
+
+void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, uint64_t* k2, uint32_t* v2, size_t n,
+                      int bits, const uint8_t* sysd, uint32_t* tmp, size_t cap, hipStream_t s)
+{
+    if (n == 0 || n > cap) return;
+    const unsigned T = static_cast<unsigned>(ns_tiles(n));
+    k_ns_mark<<<T, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
+    k_ns_plan<<<1, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
+    k_ns_scatter<<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
+    const uint32_t* ctl = ns_tmp_at(tmp, cap).ctl;
+    const int key_passes = (bits + 7) / 8;   // <= 8
+    const int passes = key_passes + (sysd ? 1 : 0);
+    const bool big = n >= kBigSort;
+    const unsigned nt = static_cast<unsigned>(sort_tiles(n, big ? 2 * kSortTile : kSortTile));
+    // pass p writes (k1, v1) for even p, (k2, v2) for odd p; pass 0 reads
+    // (k0, v0) in full mode and the removed keys (k2, v2) in near mode
+    if (big) k_sort_hist_all<16><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, k2, v2, ctl);
+    else k_sort_hist_all<8><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, k2, v2, ctl);
+    for (int p = 0; p < passes; ++p) {
+        const uint8_t* sd = p < key_passes ? nullptr : sysd;
+        const uint64_t* ka = p == 0 ? k0 : (p % 2 ? k1 : k2);
+        const uint32_t* va = p == 0 ? v0 : (p % 2 ? v1 : v2);
+        uint64_t* kb = p % 2 ? k2 : k1;
+        uint32_t* vb = p % 2 ? v2 : v1;
+        const uint64_t* kn = p == 0 ? k2 : nullptr;
+        const uint32_t* vn = p == 0 ? v2 : nullptr;
+        if (big) k_sort_pass<16><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, kn, vn, ctl);
+        else k_sort_pass<8><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, kn, vn, ctl);
+    }
+    const uint64_t* fk = (passes - 1) % 2 ? k2 : k1;
+    const uint32_t* fv = (passes - 1) % 2 ? v2 : v1;
+    k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, fk, fv, sysd, tmp, cap);
 }
 
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
@@ -437,16 +783,16 @@ void radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, si
     const int passes = key_passes + (sysd ? 1 : 0);
     const bool big = n >= kBigSort;
     const unsigned nt = static_cast<unsigned>(sort_tiles(n, big ? 2 * kSortTile : kSortTile));
-    if (big) k_sort_hist_all<16><<<nt, kNT, 0, s>>>(k0, n, key_passes, sysd, tmp);
-    else k_sort_hist_all<8><<<nt, kNT, 0, s>>>(k0, n, key_passes, sysd, tmp);
+    if (big) k_sort_hist_all<16><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, nullptr, nullptr, nullptr);
+    else k_sort_hist_all<8><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, nullptr, nullptr, nullptr);
     uint64_t* ka = k0;
     uint32_t* va = v0;
     uint64_t* kb = k1;
     uint32_t* vb = v1;
     for (int p = 0; p < passes; ++p) {
         const uint8_t* sd = p < key_passes ? nullptr : sysd;
-        if (big) k_sort_pass<16><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp);
-        else k_sort_pass<8><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp);
+        if (big) k_sort_pass<16><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, nullptr, nullptr, nullptr);
+        else k_sort_pass<8><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, nullptr, nullptr, nullptr);
         uint64_t* tk = ka;
         ka = kb;
         kb = tk;

@@ -169,6 +169,20 @@ int bh_set_schedule(bh_ctx *ctx, int mode);
  * neighbouring groups on the device at the same time.  Default 0. */
 int bh_set_schedule_split(bh_ctx *ctx, int chunks_per_worker);
 
+/* Key sort of the step (no counterpart in the reference, whose
+ * sortBodiesByMortonKey, include/utility.h:23, is a full stable sort every
+ * step).  BH_SORT_NEAR (default): the bodies stay in the previous step's
+ * order, so only the keys out of order with their neighbours are radix
+ * sorted and merged with the rest; when the rest is not in order (or more
+ * than half the keys moved) every key is sorted.  BH_SORT_FULL: always the
+ * full radix sort.  Both give the same, stable order. */
+#define BH_SORT_NEAR 0
+#define BH_SORT_FULL 1
+int bh_set_sort_mode(bh_ctx *ctx, int mode);
+/* Path of the last key sort: *near_path = 1 if it took the near-sorted path,
+ * *sorted_keys = the keys it radix sorted (n otherwise).  Synchronises. */
+int bh_get_sort_stats(bh_ctx *ctx, int *near_path, int64_t *sorted_keys);
+
 /* ------------------------------------------------ learned cost model ---
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
  * accumulation) predicts log2(list length + 1) / 16 of every walk group as
