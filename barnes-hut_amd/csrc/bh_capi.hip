@@ -255,10 +255,16 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         bh::MixedParams mp{theta * theta, G, soft2, half_dt, c->mac_alpha, kick_after ? 1 : 0,
                            sched == BH_SCHED_KEY_ORDER ? 0 : (split ? 2 : 1), n_chunks, order_max,
                            c->heavy_items};
-        bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mp, c->mixed_blocks, s);
-        // heavy bodies after the walk: run beside the tree build on a second
-        // stream they slowed the latency-bound tree kernels more than they hid
-        bh::launch_heavy_direct(c->cur, tb, c->mb, mp, s);
+        // the heavy bodies' chunk sums: at most 2 per walk wave run in the
+        // walk's tail (waves whose groups are done take them: -14 us at 300k);
+        // more (+6% walk at 2M, where the tail is short) get their own launch.
+        // Their final sums and kick follow.  Run beside the tree build on a
+        // second stream they had slowed the latency-bound tree kernels.
+        const bool heavy_in_walk = c->heavy_items <= 2u * 4u * c->mixed_blocks;
+        bh::MixedParams mw = mp;
+        if (!heavy_in_walk) mw.heavy_items = 0;
+        bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mw, c->mixed_blocks, s);
+        bh::launch_heavy_direct(c->cur, tb, c->mb, mp, heavy_in_walk, s);
         if (training) {
             // training phase: the walk's list lengths become samples, then
             // iters_per_step Adam updates on mini-batches of all samples so far.

@@ -78,6 +78,7 @@ struct StepScalars {
     uint32_t group_counter;  // dynamic work queue of the mixed walk (cost order, split schedule)
     uint32_t cost_total;     // sum of the group weights of the split schedule
     uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
+    uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
 };
 
 // In key order the mixed walk deals its groups out in runs of kWalkRun
@@ -265,8 +266,10 @@ void launch_mixed_prepare(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffe
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
                        const MixedParams& p, unsigned blocks, hipStream_t s);
 // FP64 direct sums of the heavy bodies of every system (accelerations + kick).
+// in_walk: the chunk sums were taken by the mixed walk's tail; only the final
+// sums run here.
 void launch_heavy_direct(const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, const MixedParams& p,
-                         hipStream_t s);
+                         bool in_walk, hipStream_t s);
 unsigned mixed_walk_blocks(int device);
 // Heaviest-first group order into mb.order (source 0: previous step's list
 // lengths; 1: mb.gcost from the cost model).

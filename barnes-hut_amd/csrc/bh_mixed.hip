@@ -456,8 +456,71 @@ __device__ inline MNode fetch_node(const MixedBuffers& mb, const uint32_t* nbuf,
     return nd;
 }
 
-// 135 VGPRs: 3 waves per SIMD.  Capping at 128 for 4 waves measured slower on
-// the 300k and ensemble walks (the list loop needs its registers).
+// Accelerations of the heavy bodies: an FP64 direct sum over every other
+// body (Neumaier-compensated per thread, partials combined in thread order).
+// The heavy bodies are few and their net accelerations are small next to the
+// forces that cancel in them, so no tree truncation is taken for them.
+struct Neumaier {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x)
+    {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    __device__ double value() const { return s + c; }
+};
+
+// Work item w = (system s, heavy rank k, chunk c): heavy body k of system s
+// summed over the c-th run of 1024 bodies of s by one wavefront -- each lane
+// a Neumaier sum over its stride, lane partials added in lane order -- and
+// k_heavy_final adds the chunk partials in chunk order.  A system's heavy
+// accelerations therefore depend on that system alone, not on the launch
+// shape or on the other systems of an ensemble.
+__device__ inline void heavy_item(const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, const MixedParams& p,
+                                  uint32_t item, int lane, double (*part)[64])
+{
+    uint32_t lo = 0, hi = tb.n_sys;   // the last system whose first item is <= item
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tb.sys[mid].heavy_item <= item) lo = mid;
+        else hi = mid;
+    }
+    const SysScalars* sy = tb.sys + lo;
+    const uint32_t r = item - sy->heavy_item;
+    const uint32_t k = r / sy->heavy_chunks, c = r % sy->heavy_chunks;
+    if (k >= sy->n_heavy) return;
+    const uint32_t h = sy->heavy_slot[k];
+    const double x = b.px[h], y = b.py[h], z = b.pz[h];
+    const size_t j0 = sy->body_lo + static_cast<size_t>(c) * kHeavyChunk;
+    const size_t j1 = min(j0 + kHeavyChunk, static_cast<size_t>(sy->body_hi));
+    Neumaier ax, ay, az;
+    for (size_t j = j0 + lane; j < j1; j += 64) {
+        if (j == h) continue;
+        const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
+        const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
+        if (!(r2 > 0.0)) continue;
+        const double invR = 1.0 / sqrt(r2);
+        const double s = p.G * b.m[j] * (invR * invR * invR);
+        ax.add(s * dx);
+        ay.add(s * dy);
+        az.add(s * dz);
+    }
+    part[0][lane] = ax.value();
+    part[1][lane] = ay.value();
+    part[2][lane] = az.value();
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 3) {
+        Neumaier t;
+        for (int q = 0; q < 64; ++q) t.add(part[lane][q]);
+        mb.hpart[3 * static_cast<size_t>(item) + lane] = t.value();
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// 151 VGPRs: 3 waves per SIMD.  Capping at 128 for 4 waves measured 1% slower
+// on the 300k walk and 4% faster at 2M (the list loop needs its registers).
 __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
                                                     MixedParams p)
 {
@@ -738,73 +801,25 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
             mb.g_worker[g] = worker;
         }
     }
+    // then the heavy bodies' chunk sums, in the walk's tail: a wave whose
+    // groups are done takes work items while the last groups finish (its LDS
+    // list holds the lane partials)
+    for (;;) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(&tb.sc->heavy_counter, 1u);
+        q = uni(q);
+        if (q >= p.heavy_items) break;
+        heavy_item(b, tb, mb, p, q, lane, reinterpret_cast<double(*)[64]>(list));
+    }
 }
 
-// Accelerations of the heavy bodies: an FP64 direct sum over every other
-// body (Neumaier-compensated per thread, partials combined in thread order).
-// The heavy bodies are few and their net accelerations are small next to the
-// forces that cancel in them, so no tree truncation is taken for them.
-struct Neumaier {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double x)
-    {
-        const double t = s + x;
-        if (fabs(s) >= fabs(x)) c += (s - t) + x;
-        else c += (x - t) + s;
-        s = t;
-    }
-    __device__ double value() const { return s + c; }
-};
-
-// Work item w = (system s, heavy rank k, chunk c): heavy body k of system s
-// summed over the c-th run of 1024 bodies of s by one wavefront -- each lane
-// a Neumaier sum over its stride, lane partials added in lane order -- and
-// k_heavy_final adds the chunk partials in chunk order.  A system's heavy
-// accelerations therefore depend on that system alone, not on the launch
-// shape or on the other systems of an ensemble.
 __global__ __launch_bounds__(kNT) void k_heavy_partial(Bodies b, TreeBuffers tb, MixedBuffers mb, MixedParams p)
 {
     __shared__ double part[kWaves][3][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * kWaves;
-    for (uint32_t item = blockIdx.x * kWaves + w; item < p.heavy_items; item += waves) {
-        uint32_t lo = 0, hi = tb.n_sys;   // the last system whose first item is <= item
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (tb.sys[mid].heavy_item <= item) lo = mid;
-            else hi = mid;
-        }
-        const SysScalars* sy = tb.sys + lo;
-        const uint32_t r = item - sy->heavy_item;
-        const uint32_t k = r / sy->heavy_chunks, c = r % sy->heavy_chunks;
-        if (k >= sy->n_heavy) continue;
-        const uint32_t h = sy->heavy_slot[k];
-        const double x = b.px[h], y = b.py[h], z = b.pz[h];
-        const size_t j0 = sy->body_lo + static_cast<size_t>(c) * kHeavyChunk;
-        const size_t j1 = min(j0 + kHeavyChunk, static_cast<size_t>(sy->body_hi));
-        Neumaier ax, ay, az;
-        for (size_t j = j0 + lane; j < j1; j += 64) {
-            if (j == h) continue;
-            const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
-            const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
-            if (!(r2 > 0.0)) continue;
-            const double invR = 1.0 / sqrt(r2);
-            const double s = p.G * b.m[j] * (invR * invR * invR);
-            ax.add(s * dx);
-            ay.add(s * dy);
-            az.add(s * dz);
-        }
-        part[w][0][lane] = ax.value();
-        part[w][1][lane] = ay.value();
-        part[w][2][lane] = az.value();
-        __builtin_amdgcn_wave_barrier();
-        if (lane < 3) {
-            Neumaier t;
-            for (int q = 0; q < 64; ++q) t.add(part[w][lane][q]);
-            mb.hpart[3 * static_cast<size_t>(item) + lane] = t.value();
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
+    for (uint32_t item = blockIdx.x * kWaves + w; item < p.heavy_items; item += waves)
+        heavy_item(b, tb, mb, p, item, lane, part[w]);
 }
 
 // One wavefront per (system, heavy rank, axis): lane l adds the chunk
@@ -1017,11 +1032,11 @@ void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const M
 }
 
 void launch_heavy_direct(const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, const MixedParams& p,
-                         hipStream_t s)
+                         bool in_walk, hipStream_t s)
 {
     if (p.heavy_items == 0) return;
     const unsigned hb = std::min<unsigned>((p.heavy_items + kWaves - 1) / kWaves, 4096u);
-    k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
+    if (!in_walk) k_heavy_partial<<<hb, kNT, 0, s>>>(b, tb, mb, p);
     k_heavy_final<<<(tb.n_sys * 3 * kMaxHeavy + kWaves - 1) / kWaves, kNT, 0, s>>>(b, tb, mb, p);
 }
 

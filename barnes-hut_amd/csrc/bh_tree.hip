@@ -213,6 +213,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
         sc->error = 0;
         sc->group_counter = 0;
         sc->block_counter = 0;
+        sc->heavy_counter = 0;
     }
     if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
 }
