@@ -188,8 +188,11 @@ void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s
 // system also starts a leaf); heavy slots of every system.
 void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s);
 // Tree topology and raw moments from sorted keys / bodies and first-of-key
-// flags (tb.flag); light moments too when tb.lraw is set.
-void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s);
+// flags (tb.flag); light moments too when tb.lraw is set.  moments = false
+// stops after the topology and the leaf moments; launch_tree_moments then
+// adds the branching nodes' moments (21 per-depth launches).
+void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s);
+void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s);
 // Walk-order records of the plain walk (after launch_build_tree).
 void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s);
 
@@ -259,9 +262,13 @@ struct GroupParams {
     uint32_t occ = 32;
 };
 
-// Walk groups and the children-contiguous node array (after the tree build).
-void launch_mixed_prepare(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
-                          double G, hipStream_t s);
+// Mixed forms after the tree topology: launch_mixed_layout (depth histogram
+// and extent limits, children-contiguous slots) needs no moments and may run
+// beside launch_tree_moments; launch_mixed_groups (walk groups, node array)
+// needs both.
+void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, hipStream_t s);
+void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
+                         double G, hipStream_t s);
 // Persistent mixed walk over the groups (light bodies: accelerations + kick).
 void launch_walk_mixed(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb,
                        const MixedParams& p, unsigned blocks, hipStream_t s);

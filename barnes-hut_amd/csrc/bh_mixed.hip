@@ -990,17 +990,21 @@ void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const Mixed
     k_chunk_bounds<<<(n_chunks + kNT) / kNT, kNT, 0, s>>>(tb, mb, mb.base, n_chunks);
 }
 
-void launch_mixed_prepare(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
-                          double G, hipStream_t s)
+void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, hipStream_t s)
 {
     k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb);
     k_sys_extent<<<1, kNT, 0, s>>>(tb, gp);
-    k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
-    scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
-    k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
     k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, mb.order);
     scan_exclusive_u32(mb.order, mb.base, nullptr, n, tb.scan_tmp, s);
     k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+}
+
+void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
+                         double G, hipStream_t s)
+{
+    k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
+    scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
+    k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
     k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, b, tb, mb, G);
 }
 

@@ -597,7 +597,12 @@ void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb,
     k_permute<<<nblk(n), kNT, 0, s>>>(src, dst, tb, n);
 }
 
-void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s)
+void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s)
+{
+    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
+}
+
+void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s)
 {
     // leaf index of every body: exclusive scan of first-of-key flags (M total)
     scan_exclusive_u32(tb.flag, tb.leaf_of, &tb.sc->n_leaves, n, tb.scan_tmp, s);
@@ -606,7 +611,7 @@ void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, hipStream_t s
     scan_exclusive_u32(tb.is_rep, tb.bidx, &tb.sc->n_branch, n, tb.scan_tmp, s);
     k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
     k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
-    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
+    if (moments) launch_tree_moments(n, tb, s);
 }
 
 void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s)

@@ -98,11 +98,13 @@ def test_ensemble_steps_match_single_runs(bh):
             assert np.array_equal(bits(st[key][sl]), bits(r[key])), (k, key)
 
 
-@pytest.mark.parametrize("form", ["mixed", "learned"])
-def test_graph_replay_matches_eager(bh, form):
+@pytest.mark.parametrize("form,n", [("mixed", 40000), ("learned", 40000), ("mixed", 1 << 20)])
+def test_graph_replay_matches_eager(bh, form, n):
+    """From 2^20 bodies the tree phase forks onto a second stream, so the
+    captured graph has two branches; replays must still match eager steps."""
     f = bh.FORM_MIXED if form == "mixed" else bh.FORM_LEARNED
-    say("graph vs eager", form)
-    pos, vel, m = bh.generate(bh.GEN_TROJAN, 5, 40000)
+    say("graph vs eager", form, n)
+    pos, vel, m = bh.generate(bh.GEN_TROJAN, 5, n)
     engines = []
     for graph in (False, True):
         e = bh.Engine(form=f, theta=1.05)
