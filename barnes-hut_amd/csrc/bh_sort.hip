@@ -461,13 +461,9 @@ __host__ __device__ inline size_t ns_tiles(size_t n) { return (n + kNsTile - 1) 
 struct NsTmp {
     uint32_t* ctl;     // 8
     uint32_t* tcnt;    // kept keys per tile
-    uint32_t* tfirst;  // index of the tile's first kept key (kNone: none)
-    uint32_t* tlast;   // index of its last kept key
-    uint32_t* tok;     // 1: the tile's kept keys are in order among themselves
-    uint32_t* toff;    // exclusive prefix of tcnt
+    uint32_t* tok;     // 1: every kept key of the tile is in order with the kept key before it
     uint8_t* flag;     // 1: kept
 };
-constexpr uint32_t kNone32 = 0xffffffffu;
 
 __host__ __device__ inline NsTmp ns_tmp_at(uint32_t* tmp, size_t cap)
 {
@@ -475,171 +471,148 @@ __host__ __device__ inline NsTmp ns_tmp_at(uint32_t* tmp, size_t cap)
     const size_t T = ns_tiles(cap);
     t.ctl = sort_tmp_at(tmp).chain + kChainHeadWords + 2 * kDigits * sort_tiles(cap);
     t.tcnt = t.ctl + 8;
-    t.tfirst = t.tcnt + T;
-    t.tlast = t.tfirst + T;
-    t.tok = t.tlast + T;
-    t.toff = t.tok + T;
-    t.flag = reinterpret_cast<uint8_t*>(t.toff + T);
+    t.tok = t.tcnt + T;
+    t.flag = reinterpret_cast<uint8_t*>(t.tok + T);
     return t;
 }
-__host__ __device__ inline size_t ns_tmp_words(size_t cap) { return 8 + 5 * ns_tiles(cap) + (cap + 3) / 4; }
+__host__ __device__ inline size_t ns_tmp_words(size_t cap) { return 8 + 2 * ns_tiles(cap) + (cap + 3) / 4; }
 
 // a at index i before b at index j > i is in order
 __device__ inline bool ns_in_order(uint64_t ka, uint8_t sa, uint64_t kb, uint8_t sb) { return sa != sb || ka <= kb; }
 
-// Kept flags of one tile of 2048 keys (8 per thread, thread-strided) and the
-// tile's kept count, first and last kept index and inner order.
+// Kept flags of one tile of 2048 keys (8 per thread, thread-strided), its
+// kept count, and whether each kept key is in order with the kept key before
+// it.  The tile also decides the flags of the kNsBackScan keys before it (a
+// halo of keys, so no other block's result is needed): a kept key whose kept
+// predecessor lies more than kNsWin places back, in this tile or the one
+// before, is compared with it directly; a run of more than kNsBackScan
+// removed keys fails the tile.
+constexpr int kNsLeft = kNsBackScan + kNsWin;   // halo keys before the tile
 __global__ __launch_bounds__(kNT) void k_ns_mark(const uint64_t* __restrict__ key, size_t n,
                                                  const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
 {
-    __shared__ uint64_t sk[kNsTile + 2 * kNsWin];
-    __shared__ uint8_t ss[kNsTile + 2 * kNsWin];
-    __shared__ uint8_t sf[kNsTile];
-    __shared__ uint32_t r_cnt[kWaves], r_first[kWaves], r_last[kWaves], r_ok[kWaves];
+    constexpr int kSpan = kNsLeft + kNsTile + kNsWin;
+    __shared__ uint64_t sk[kSpan];
+    __shared__ uint8_t ss[kSpan];
+    __shared__ uint8_t sf[kNsBackScan + kNsTile];   // flags of [t0 - kNsBackScan, t0 + kNsTile)
+    __shared__ uint32_t r_cnt[kWaves], r_ok[kWaves];
     const NsTmp nt = ns_tmp_at(tmp, cap);
     const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kNsTile;
-    for (int e = threadIdx.x; e < kNsTile + 2 * kNsWin; e += kNT) {
-        const int64_t i = t0 + e - kNsWin;
-        const bool in = i >= 0 && i < static_cast<int64_t>(n);
+    const int64_t nn = static_cast<int64_t>(n);
+    for (int e = threadIdx.x; e < kSpan; e += kNT) {
+        const int64_t i = t0 + e - kNsLeft;
+        const bool in = i >= 0 && i < nn;
         sk[e] = in ? key[i] : 0ull;
         ss[e] = (in && sysd) ? sysd[i] : 0;
     }
     __syncthreads();
-    uint32_t cnt = 0, first = kNone32, last = 0;
-#pragma unroll
-    for (int j = 0; j < kNsIPT; ++j) {
-        const int e = j * kNT + threadIdx.x;   // tile-relative index
+    // the window rule for index i (tile-relative e = i - t0 >= -kNsBackScan)
+    auto kept = [&](int e) {
         const int64_t i = t0 + e;
-        bool good = false;
-        if (i < static_cast<int64_t>(n)) {
-            good = true;
-            const int x = e + kNsWin;
+        if (i < 0 || i >= nn) return false;
+        const int x = e + kNsLeft;
+        bool good = true;
 #pragma unroll
-            for (int d = 1; d <= kNsWin; ++d) {
-                if (i - d >= 0) good = good && ns_in_order(sk[x - d], ss[x - d], sk[x], ss[x]);
-                if (i + d < static_cast<int64_t>(n)) good = good && ns_in_order(sk[x], ss[x], sk[x + d], ss[x + d]);
-            }
-            nt.flag[i] = good ? 1 : 0;
-            if (good) {
-                ++cnt;
-                first = min(first, static_cast<uint32_t>(i));
-                last = max(last, static_cast<uint32_t>(i));
-            }
+        for (int d = 1; d <= kNsWin; ++d) {
+            if (i - d >= 0) good = good && ns_in_order(sk[x - d], ss[x - d], sk[x], ss[x]);
+            if (i + d < nn) good = good && ns_in_order(sk[x], ss[x], sk[x + d], ss[x + d]);
         }
-        sf[e] = good ? 1 : 0;
+        return good;
+    };
+    uint32_t cnt = 0;
+    for (int e = static_cast<int>(threadIdx.x) - kNsBackScan; e < kNsTile; e += kNT) {
+        const bool g = kept(e);
+        sf[e + kNsBackScan] = g ? 1 : 0;
+        if (e >= 0) {
+            if (t0 + e < nn) nt.flag[t0 + e] = g ? 1 : 0;
+            cnt += g ? 1u : 0u;
+        }
     }
     __syncthreads();
-    // a kept key whose kept predecessor in the tile lies beyond the window
-    // is compared with it directly (runs longer than kNsBackScan fail)
     bool ok = true;
-#pragma unroll
-    for (int j = 0; j < kNsIPT; ++j) {
-        const int e = j * kNT + threadIdx.x;
-        if (!sf[e]) continue;
+    for (int e = threadIdx.x; e < kNsTile; e += kNT) {
+        if (!sf[e + kNsBackScan]) continue;
         int q = e - 1, steps = 0;
-        while (q >= 0 && !sf[q] && steps < kNsBackScan) {
+        while (q >= -kNsBackScan && t0 + q >= 0 && !sf[q + kNsBackScan] && steps < kNsBackScan) {
             --q;
             ++steps;
         }
-        if (q >= 0 && !sf[q]) ok = false;   // gave up
-        else if (q >= 0 && e - q > kNsWin)
-            ok = ok && ns_in_order(sk[q + kNsWin], ss[q + kNsWin], sk[e + kNsWin], ss[e + kNsWin]);
+        if (t0 + q < 0) continue;   // no key before it
+        if (q < -kNsBackScan || !sf[q + kNsBackScan]) ok = false;   // gave up
+        else if (e - q > kNsWin)
+            ok = ok && ns_in_order(sk[q + kNsLeft], ss[q + kNsLeft], sk[e + kNsLeft], ss[e + kNsLeft]);
     }
     uint32_t okw = ok ? 1u : 0u;
     for (int o = 32; o > 0; o >>= 1) {
         cnt += static_cast<uint32_t>(__shfl_xor(static_cast<int>(cnt), o, 64));
-        first = min(first, static_cast<uint32_t>(__shfl_xor(static_cast<int>(first), o, 64)));
-        last = max(last, static_cast<uint32_t>(__shfl_xor(static_cast<int>(last), o, 64)));
         okw &= static_cast<uint32_t>(__shfl_xor(static_cast<int>(okw), o, 64));
     }
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         r_cnt[w] = cnt;
-        r_first[w] = first;
-        r_last[w] = last;
         r_ok[w] = okw;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t c = 0, f = kNone32, l = 0, k = 1;
+        uint32_t c = 0, k = 1;
         for (int q = 0; q < kWaves; ++q) {
             c += r_cnt[q];
-            f = min(f, r_first[q]);
-            l = max(l, r_last[q]);
             k &= r_ok[q];
         }
         nt.tcnt[blockIdx.x] = c;
-        nt.tfirst[blockIdx.x] = f;
-        nt.tlast[blockIdx.x] = l;
         nt.tok[blockIdx.x] = k;
     }
 }
 
-// One block: tile offsets, the order across tiles, the mode.
-__global__ __launch_bounds__(kNT) void k_ns_plan(const uint64_t* __restrict__ key, size_t n,
-                                                 const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
-{
-    __shared__ uint32_t lds[kWaves + 1];
-    __shared__ uint32_t s_last[kNT];
-    __shared__ uint32_t s_bad;
-    const NsTmp nt = ns_tmp_at(tmp, cap);
-    const uint32_t T = static_cast<uint32_t>(ns_tiles(n));
-    if (threadIdx.x == 0) s_bad = 0;
-    uint32_t carry = 0, carry_last = kNone32;   // kept so far; last kept index of the tiles so far
-    for (uint32_t c0 = 0; c0 < T; c0 += kNT) {
-        const uint32_t t = c0 + threadIdx.x;
-        const bool in = t < T;
-        const uint32_t cnt = in ? nt.tcnt[t] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(cnt, &total, lds);
-        if (in) nt.toff[t] = carry + ex;
-        // the nearest kept key before this tile: the largest last-kept index
-        // of the earlier tiles (indices grow with the tile)
-        s_last[threadIdx.x] = (in && cnt) ? nt.tlast[t] : kNone32;
-        __syncthreads();
-        uint32_t prev = carry_last;
-        for (int k = static_cast<int>(threadIdx.x) - 1; k >= 0; --k)
-            if (s_last[k] != kNone32) {
-                prev = s_last[k];
-                break;
-            }
-        bool bad = in && !nt.tok[t];
-        if (in && cnt && prev != kNone32) {
-            const uint32_t f = nt.tfirst[t];
-            const uint8_t sa = sysd ? sysd[prev] : 0, sb = sysd ? sysd[f] : 0;
-            if (!ns_in_order(key[prev], sa, key[f], sb)) bad = true;
-        }
-        if (bad) atomicOr(&s_bad, 1u);
-        for (int k = kNT - 1; k >= 0; --k)
-            if (s_last[k] != kNone32) {
-                carry_last = s_last[k];
-                break;
-            }
-        carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const uint32_t kept = carry, removed = static_cast<uint32_t>(n) - kept;
-        const bool near = !s_bad && removed <= static_cast<uint32_t>(n / 2);
-        nt.ctl[0] = near ? 1u : 0u;
-        nt.ctl[1] = near ? removed : static_cast<uint32_t>(n);
-        nt.ctl[2] = near ? kept : 0u;
-        nt.ctl[3] = nt.ctl[1];
-        nt.ctl[5] = removed;
-    }
-}
-
-// Near mode: kept keys (in index order) to (k1, v1) + R, removed ones to
-// (k2, v2), both with their payloads.
+// Every block adds the kept counts of the tiles before its own and of all
+// tiles, and the tiles' checks, so all blocks reach the same mode; block 0
+// writes the control words.  Near mode: kept keys (in index order) to
+// (k1, v1) + R, removed ones to (k2, v2), both with their payloads.
 __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__ k0, const uint32_t* __restrict__ v0,
                                                     size_t n, uint64_t* __restrict__ k1, uint32_t* __restrict__ v1,
                                                     uint64_t* __restrict__ k2, uint32_t* __restrict__ v2,
                                                     uint32_t* tmp, size_t cap)
 {
     __shared__ uint32_t lds[kWaves + 1];
+    __shared__ uint32_t r_pre[kWaves], r_tot[kWaves], r_ok[kWaves];
     const NsTmp nt = ns_tmp_at(tmp, cap);
-    if (nt.ctl[0] != 1u) return;
-    const uint32_t R = nt.ctl[3];
+    const uint32_t T = static_cast<uint32_t>(ns_tiles(n));
+    uint32_t pre = 0, tot = 0, ok = 1;
+    for (uint32_t u = threadIdx.x; u < T; u += kNT) {
+        const uint32_t c = nt.tcnt[u];
+        tot += c;
+        if (u < blockIdx.x) pre += c;
+        ok &= nt.tok[u];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        pre += static_cast<uint32_t>(__shfl_xor(static_cast<int>(pre), o, 64));
+        tot += static_cast<uint32_t>(__shfl_xor(static_cast<int>(tot), o, 64));
+        ok &= static_cast<uint32_t>(__shfl_xor(static_cast<int>(ok), o, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        r_pre[w] = pre;
+        r_tot[w] = tot;
+        r_ok[w] = ok;
+    }
+    __syncthreads();
+    pre = tot = 0;
+    ok = 1;
+    for (int q = 0; q < kWaves; ++q) {
+        pre += r_pre[q];
+        tot += r_tot[q];
+        ok &= r_ok[q];
+    }
+    const uint32_t R = static_cast<uint32_t>(n) - tot;
+    const bool near = ok && R <= static_cast<uint32_t>(n / 2);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        nt.ctl[0] = near ? 1u : 0u;
+        nt.ctl[1] = near ? R : static_cast<uint32_t>(n);
+        nt.ctl[2] = near ? tot : 0u;
+        nt.ctl[3] = nt.ctl[1];
+        nt.ctl[5] = R;
+    }
+    if (!near) return;
     const size_t base = static_cast<size_t>(blockIdx.x) * kNsTile + static_cast<size_t>(threadIdx.x) * kNsIPT;
     uint32_t f[kNsIPT], cnt = 0;
 #pragma unroll
@@ -648,7 +621,7 @@ __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__
         cnt += f[j];
     }
     uint32_t total;
-    uint32_t kept_before = nt.toff[blockIdx.x] + block_exclusive_scan(cnt, &total, lds);
+    uint32_t kept_before = pre + block_exclusive_scan(cnt, &total, lds);
 #pragma unroll
     for (int j = 0; j < kNsIPT; ++j) {
         const size_t i = base + j;
@@ -746,7 +719,6 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     if (n == 0 || n > cap) return;
     const unsigned T = static_cast<unsigned>(ns_tiles(n));
     k_ns_mark<<<T, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
-    k_ns_plan<<<1, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
     k_ns_scatter<<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
     const uint32_t* ctl = ns_tmp_at(tmp, cap).ctl;
     const int key_passes = (bits + 7) / 8;   // <= 8
