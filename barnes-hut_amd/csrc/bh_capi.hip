@@ -258,12 +258,15 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         // above this many groups the walk serves key order and the order
         // kernels skip their work (training still needs the features)
         const uint32_t order_max = split ? 0xffffffffu : 16u * n_workers;
-        if (learned || sched == BH_SCHED_MODEL) {
-            bh::launch_group_features(c->n, c->cur, tb, c->mb, c->md.feat, training ? 0xffffffffu : order_max, s);
+        const bool want_features = learned || sched == BH_SCHED_MODEL;
+        if (sched == BH_SCHED_MODEL && !c->model_ready) sched = BH_SCHED_PREV_COST;
+        if (want_features) {
+            // with a ready model the features kernel also predicts the costs
+            // and counts the groups into the order's buckets
+            bh::launch_group_features(c->n, c->cur, tb, c->mb, c->md.feat, training ? 0xffffffffu : order_max,
+                                      sched == BH_SCHED_MODEL ? c->md.wbf : nullptr, s);
             c->have_features = true;
         }
-        if (sched == BH_SCHED_MODEL && !c->model_ready) sched = BH_SCHED_PREV_COST;
-        if (sched == BH_SCHED_MODEL) bh::launch_model_cost(c->n, c->md.feat, c->md, tb, c->mb, order_max, s);
         if (split)
             bh::launch_chunks(c->cur, c->n, tb, c->mb, sched == BH_SCHED_MODEL ? 1 : 0, n_chunks, s);
         else if (sched != BH_SCHED_KEY_ORDER)

@@ -79,6 +79,7 @@ struct StepScalars {
     uint32_t cost_total;     // sum of the group weights of the split schedule
     uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
     uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
+    uint32_t sched_done;        // last-block-done counter of the schedule histogram, left at 0
 };
 
 // In key order the mixed walk deals its groups out in runs of kWalkRun
@@ -243,6 +244,46 @@ struct MixedBuffers {
     uint32_t* sched_cursor;  // 64 per part
 };
 
+// ---------------------------------------------------- schedule histogram
+// Heaviest-first order: 64 buckets of the estimated list length c, two per
+// octave, bucket 0 the costliest.
+__device__ inline uint32_t cost_bucket(float c)
+{
+    const float l = 2.f * __log2f(c + 1.f);
+    const int k = static_cast<int>(l);
+    return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
+}
+
+// After a block's bucket counts in LDS (hist[64]) are complete: adds them to
+// the global counts; the last of `blocks` blocks to arrive turns the counts
+// into exclusive bucket offsets (sched_cursor) and clears the counts and the
+// counter for the next step.  Every thread of the block calls it.
+__device__ inline void sched_hist_commit(const TreeBuffers& tb, const MixedBuffers& mb, const uint32_t* hist,
+                                         uint32_t blocks, bool* last)
+{
+    __syncthreads();
+    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&mb.sched_hist[threadIdx.x], hist[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        *last = atomicAdd(&tb.sc->sched_done, 1u) == blocks - 1;
+    }
+    __syncthreads();
+    if (!*last) return;
+    __threadfence();
+    if (threadIdx.x < 64) {
+        const uint32_t v = __hip_atomic_load(&mb.sched_hist[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t inc = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+            if (static_cast<int>(threadIdx.x) >= o) inc += t;
+        }
+        mb.sched_cursor[threadIdx.x] = inc - v;
+        __hip_atomic_store(&mb.sched_hist[threadIdx.x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(&tb.sc->sched_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct MixedParams {
     double theta2, G, soft2, half_dt;
     double alpha;         // relative opening test G Q <= alpha |a_heavy| D^4 (0: off)
@@ -310,10 +351,11 @@ struct ModelBuffers {
 };
 constexpr int kMaxTrainBatch = 1 << 16;   // samples per gradient evaluation
 
+// Group features; with the model's BF16 weights (model != nullptr) also the
+// predicted costs (mb.gcost) and the bucket offsets of the heaviest-first
+// order, so launch_schedule(source 1) only places the groups.
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
-                           uint32_t max_groups, hipStream_t s);
-void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
-                       const MixedBuffers& mb, uint32_t max_groups, hipStream_t s);
+                           uint32_t max_groups, const uint16_t* model, hipStream_t s);
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s);
 void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, const MixedBuffers& mb,
                            const ModelBuffers& md, hipStream_t s);

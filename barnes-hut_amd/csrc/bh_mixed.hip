@@ -858,53 +858,35 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
 //
 // Cost estimate, source 0: the previous step, i.e. the largest list length
 // any of the group's bodies saw (groups mostly persist from step to step).
-// Source 1: mb.gcost, written by the cost model.
-__device__ inline uint32_t cost_bucket(float c)
-{
-    const float l = 2.f * __log2f(c + 1.f);
-    const int k = static_cast<int>(l);
-    return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
-}
-
+// Source 1: the cost model, evaluated by k_group_features.
 constexpr unsigned kSchedBlocks = 256;
 
-// Source 0: one wavefront per group takes the largest previous list length of
-// its bodies.  Then every block counts its groups per bucket in LDS and adds
-// the counts to the global histogram once.
-__global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, int source,
+// Cost estimate from the previous step: one wavefront per group takes the
+// largest previous list length of its bodies and counts the group in its
+// block's bucket histogram; the last block scans the buckets
+// (sched_hist_commit).  The LEARNED form's estimate comes from
+// k_group_features instead, which ends the same way.
+__global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
                                                     uint32_t max_groups)
 {
     __shared__ uint32_t hist[64];
+    __shared__ bool last;
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
     if (threadIdx.x < 64) hist[threadIdx.x] = 0;
-    if (source == 0) {
-        const uint32_t waves = gridDim.x * kWaves;
-        for (uint32_t g = blockIdx.x * kWaves + (threadIdx.x >> 6); g < G; g += waves) {
-            const uint32_t i0 = mb.gfirst[g], cnt = mb.gfirst[g + 1] - i0;
-            uint32_t m = static_cast<uint32_t>(lane) < cnt ? b.pcost[i0 + lane] : 0u;
-            for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 64)));
-            if (lane == 0) mb.gcost[g] = static_cast<float>(m);
+    __syncthreads();
+    const uint32_t waves = gridDim.x * kWaves;
+    for (uint32_t g = blockIdx.x * kWaves + (threadIdx.x >> 6); g < G; g += waves) {
+        const uint32_t i0 = mb.gfirst[g], cnt = mb.gfirst[g + 1] - i0;
+        uint32_t m = static_cast<uint32_t>(lane) < cnt ? b.pcost[i0 + lane] : 0u;
+        for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 64)));
+        if (lane == 0) {
+            mb.gcost[g] = static_cast<float>(m);
+            atomicAdd(&hist[cost_bucket(static_cast<float>(m))], 1u);
         }
-        // the histogram below reads gcost of groups written by other blocks
-        return;
     }
-    __syncthreads();
-    for (uint32_t g = blockIdx.x * kNT + threadIdx.x; g < G; g += gridDim.x * kNT)
-        atomicAdd(&hist[cost_bucket(mb.gcost[g])], 1u);
-    __syncthreads();
-    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&mb.sched_hist[threadIdx.x], hist[threadIdx.x]);
-}
-
-__global__ __launch_bounds__(64) void k_sched_scan(MixedBuffers mb)
-{
-    // exclusive scan of the 64 bucket counts; the counts are left at zero
-    const int k = threadIdx.x;
-    const uint32_t v = mb.sched_hist[k];
-    const uint32_t inc = wave_incl_scan(v);
-    mb.sched_cursor[k] = inc - v;
-    mb.sched_hist[k] = 0;
+    sched_hist_commit(tb, mb, hist, gridDim.x, &last);
 }
 
 // Every block places a tile of 256 groups: LDS counters rank the tile's groups
@@ -982,7 +964,7 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
                    uint32_t n_chunks, hipStream_t s)
 {
-    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0, 0xffffffffu);   // previous list lengths
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0xffffffffu);   // previous list lengths
     // weights into mb.order, their exclusive prefix into mb.base (both free
     // once the node array is built)
     k_chunk_weights<<<nblk(n), kNT, 0, s>>>(n, tb, mb, mb.order);
@@ -1022,9 +1004,8 @@ unsigned mixed_walk_blocks(int device)
 void launch_schedule(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
                      uint32_t max_groups, hipStream_t s)
 {
-    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0, max_groups);   // previous list lengths
-    k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 1, max_groups);                    // bucket histogram
-    k_sched_scan<<<1, 64, 0, s>>>(mb);
+    // source 1: k_group_features has already estimated and counted the groups
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, max_groups);
     k_sched_scatter<<<kSchedBlocks, kNT, 0, s>>>(n, tb, mb, max_groups);
 }
 

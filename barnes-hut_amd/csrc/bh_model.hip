@@ -85,13 +85,56 @@ __device__ inline T wave_reduce(T v, Op op)
     return v;
 }
 
-// One wavefront per group (grid-stride): lanes hold the group's bodies.
-__global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
-                                                        float* __restrict__ feat, uint32_t max_groups)
+__device__ inline float lane_f32(float v, int k)
 {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// forward() spread over a wavefront for one group (x: the same features in
+// every lane): lane j < 32 computes hidden unit j of each layer, the output
+// runs over the units in order -- every sum in forward()'s order, so the
+// same bits.
+__device__ inline float forward_wave(const float* __restrict__ P, const float* x, int lane)
+{
+    float z1 = 0.f, z2 = 0.f;
+    if (lane < kH) {
+        float s = P[kOffB1 + lane];
+#pragma unroll
+        for (int f = 0; f < kF; ++f) s = fmaf(P[kOffW1 + lane * kF + f], x[f], s);
+        z1 = s;
+    }
+    {
+        float s = lane < kH ? P[kOffB2 + lane] : 0.f;
+        for (int k = 0; k < kH; ++k) {
+            const float a = relu(lane_f32(z1, k));
+            if (lane < kH) s = fmaf(P[kOffW2 + lane * kH + k], a, s);
+        }
+        z2 = s;
+    }
+    float y = P[kOffB3];
+    for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(lane_f32(z2, j)), y);
+    return x[5] + y;
+}
+
+// One wavefront per group (grid-stride): lanes hold the group's bodies.  With
+// the model's weights (wb), the wavefront also evaluates the model on the
+// group's features (gcost = 2^(16 y) - 1, the predicted list length), counts
+// the group in its block's bucket histogram, and the last block scans the
+// buckets for the heaviest-first order (sched_hist_commit).
+__global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
+                                                        float* __restrict__ feat, uint32_t max_groups,
+                                                        const uint16_t* __restrict__ wb)
+{
+    __shared__ float P[kModelParams];
+    __shared__ uint32_t hist[64];
+    __shared__ bool last;
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
+    if (wb) {
+        if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+        load_params(wb, P);   // ends with a barrier
+    }
     const uint32_t waves = gridDim.x * (kNT / 64);
     auto dmin = [](double a, double c) { return c < a ? c : a; };
     auto dmax = [](double a, double c) { return a < c ? c : a; };
@@ -109,12 +152,12 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
         const uint32_t pc = b.pcost[i];
         const uint32_t cmax = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0u), umax));
         const uint32_t cmin = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0xffffffffu), umin));
-        if (lane != 0) continue;
+        // every lane forms the features (wave-uniform values); lane 0 stores them
         const double ex = hx - lx, ey = hy - ly, ez = hz - lz;
         const double ext = fmax(ex, fmax(ey, ez));
         const double cx = 0.5 * (lx + hx), cy = 0.5 * (ly + hy), cz = 0.5 * (lz + hz);
         const double dist = sqrt(cx * cx + cy * cy + cz * cz);
-        float* f = feat + kF * static_cast<size_t>(g);
+        float f[kF];
         f[0] = static_cast<float>(mb.gdepth[i0]) / 21.f;
         f[1] = static_cast<float>(cnt) / 64.f;
         f[2] = flog2(static_cast<float>(ext) + 1e-12f) / 16.f;
@@ -123,27 +166,21 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
         f[5] = flog2(static_cast<float>(cmax) + 1.f) / 16.f;
         f[6] = flog2(static_cast<float>(cmin) + 1.f) / 16.f;
         f[7] = flog2(static_cast<float>(ez / (ext + 1e-30)) + 1e-6f) / 8.f;
+        if (lane == 0)
+            for (int k = 0; k < kF; ++k) feat[kF * static_cast<size_t>(g) + k] = f[k];
+        if (wb) {
+            const float y = forward_wave(P, f, lane);
+            if (lane == 0) {
+                const float c = fmaxf(exp2f(16.f * fminf(y, 1.5f)) - 1.f, 0.f);
+                mb.gcost[g] = c;
+                atomicAdd(&hist[cost_bucket(c)], 1u);
+            }
+        }
     }
+    if (wb) sched_hist_commit(tb, mb, hist, gridDim.x, &last);
 }
 
 // ------------------------------------------------------------ inference
-// gcost = 2^(16 y) - 1, the predicted list length of each group.
-__global__ __launch_bounds__(kNT) void k_model_cost(const float* __restrict__ feat, const uint16_t* __restrict__ wb,
-                                                    TreeBuffers tb, MixedBuffers mb, uint32_t max_groups)
-{
-    __shared__ float P[kModelParams];
-    if (tb.sc->n_groups > max_groups) return;   // uniform: the whole grid returns
-    load_params(wb, P);
-    const size_t g = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (g >= tb.sc->n_groups) return;
-    float x[kF];
-#pragma unroll
-    for (int f = 0; f < kF; ++f) x[f] = feat[kF * g + f];
-    Act a;
-    const float y = forward(P, x, a);
-    mb.gcost[g] = fmaxf(exp2f(16.f * fminf(y, 1.5f)) - 1.f, 0.f);
-}
-
 __global__ __launch_bounds__(kNT) void k_model_forward(const float* __restrict__ feat, size_t m,
                                                        const uint16_t* __restrict__ wb, float* __restrict__ out)
 {
@@ -344,15 +381,9 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 }  // namespace
 
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
-                           uint32_t max_groups, hipStream_t s)
+                           uint32_t max_groups, const uint16_t* model, hipStream_t s)
 {
-    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups);
-}
-
-void launch_model_cost(size_t n, const float* feat, const ModelBuffers& md, const TreeBuffers& tb,
-                       const MixedBuffers& mb, uint32_t max_groups, hipStream_t s)
-{
-    k_model_cost<<<nblk(n), kNT, 0, s>>>(feat, md.wbf, tb, mb, max_groups);
+    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups, model);
 }
 
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s)
