@@ -223,10 +223,8 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     }
 }
 
-__global__ __launch_bounds__(kNT) void k_group_first(size_t n, TreeBuffers tb, MixedBuffers mb)
+__device__ inline void group_first(size_t i, size_t n, const TreeBuffers& tb, const MixedBuffers& mb)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (i >= n) return;
     if (tb.gflag[i]) mb.gfirst[mb.gid[i]] = static_cast<uint32_t>(i);
     if (i == 0) mb.gfirst[tb.sc->n_groups] = static_cast<uint32_t>(n);
 }
@@ -260,9 +258,11 @@ __global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBu
         if (ch[o] >= 0) mb.slot[ch[o]] = b0 + k++;
 }
 
+// thread t: the first body of group t (t < n) and node record t (t < M + B)
 __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, double G)
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (t < n) group_first(t, n, tb, mb);
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
     if (t >= static_cast<size_t>(M) + B) return;
     const double* q = tb.lraw + 4 * t;
@@ -986,7 +986,6 @@ void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffer
 {
     k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
     scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
-    k_group_first<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
     k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, b, tb, mb, G);
 }
 
