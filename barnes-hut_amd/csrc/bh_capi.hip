@@ -176,11 +176,9 @@ int reserve(bh_ctx* c, size_t n)
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
     A(m.g_inter, N); A(m.g_visit, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
-    A(m.sched_hist, 64); A(m.sched_cursor, 64);
     c->heavy_item_cap = static_cast<size_t>(bh::kMaxHeavy) * (N / bh::kHeavyChunk + bh::kMaxSystems + 1);
     A(m.hpart, 3 * c->heavy_item_cap);
     A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
-    HIP_TRY(c, hipMemsetAsync(m.sched_hist, 0, 64 * sizeof(uint32_t), c->stream));
     A(t.scan_tmp, bh::scan_tmp_words(2 * N)); A(t.sort_tmp, bh::sort_tmp_words(N));
     HIP_TRY(c, hipMemsetAsync(t.scan_tmp, 0, bh::scan_tmp_words(2 * N) * sizeof(uint32_t), c->stream));
     HIP_TRY(c, hipMemsetAsync(t.sort_tmp, 0, bh::sort_tmp_words(N) * sizeof(uint32_t), c->stream));

@@ -79,7 +79,8 @@ struct StepScalars {
     uint32_t cost_total;     // sum of the group weights of the split schedule
     uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
     uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
-    uint32_t sched_done;        // last-block-done counter of the schedule histogram, left at 0
+    uint32_t sched_hist[64];    // groups per cost bucket of the heaviest-first order
+    uint32_t sched_cursor[64];  // groups placed per bucket so far (both cleared by k_kick_drift_box)
 };
 
 // In key order the mixed walk deals its groups out in runs of kWalkRun
@@ -240,8 +241,6 @@ struct MixedBuffers {
     uint32_t* chunk;    // split schedule: first group of each chunk (n_chunks + 1, <= 65536)
     float* gcost;       // N: estimated cost of each group
     double* hpart;      // heavy direct sums: 3 partials per work item (system, heavy rank, chunk)
-    uint32_t* sched_hist;    // 64 cost buckets per group part (left at zero)
-    uint32_t* sched_cursor;  // 64 per part
 };
 
 // ---------------------------------------------------- schedule histogram
@@ -254,36 +253,13 @@ __device__ inline uint32_t cost_bucket(float c)
     return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
 }
 
-// After a block's bucket counts in LDS (hist[64]) are complete: adds them to
-// the global counts; the last of `blocks` blocks to arrive turns the counts
-// into exclusive bucket offsets (sched_cursor) and clears the counts and the
-// counter for the next step.  Every thread of the block calls it.
-__device__ inline void sched_hist_commit(const TreeBuffers& tb, const MixedBuffers& mb, const uint32_t* hist,
-                                         uint32_t blocks, bool* last)
+// Adds a block's bucket counts (LDS hist[64], complete) to the step's
+// counts; every thread of the block calls it.
+__device__ inline void sched_hist_add(const TreeBuffers& tb, const uint32_t* hist)
 {
     __syncthreads();
-    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&mb.sched_hist[threadIdx.x], hist[threadIdx.x]);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        *last = atomicAdd(&tb.sc->sched_done, 1u) == blocks - 1;
-    }
-    __syncthreads();
-    if (!*last) return;
-    __threadfence();
-    if (threadIdx.x < 64) {
-        const uint32_t v = __hip_atomic_load(&mb.sched_hist[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t inc = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
-            if (static_cast<int>(threadIdx.x) >= o) inc += t;
-        }
-        mb.sched_cursor[threadIdx.x] = inc - v;
-        __hip_atomic_store(&mb.sched_hist[threadIdx.x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(&tb.sc->sched_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&tb.sc->sched_hist[threadIdx.x], hist[threadIdx.x]);
 }
-
 struct MixedParams {
     double theta2, G, soft2, half_dt;
     double alpha;         // relative opening test G Q <= alpha |a_heavy| D^4 (0: off)

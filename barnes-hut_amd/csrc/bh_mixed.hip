@@ -863,14 +863,12 @@ constexpr unsigned kSchedBlocks = 256;
 
 // Cost estimate from the previous step: one wavefront per group takes the
 // largest previous list length of its bodies and counts the group in its
-// block's bucket histogram; the last block scans the buckets
-// (sched_hist_commit).  The LEARNED form's estimate comes from
-// k_group_features instead, which ends the same way.
+// block's bucket histogram, added to the step's counts.  The LEARNED form's
+// estimate comes from k_group_features instead, which ends the same way.
 __global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
                                                     uint32_t max_groups)
 {
     __shared__ uint32_t hist[64];
-    __shared__ bool last;
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
@@ -886,17 +884,23 @@ __global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuff
             atomicAdd(&hist[cost_bucket(static_cast<float>(m))], 1u);
         }
     }
-    sched_hist_commit(tb, mb, hist, gridDim.x, &last);
+    sched_hist_add(tb, hist);
 }
 
-// Every block places a tile of 256 groups: LDS counters rank the tile's groups
+// Every block scans the step's 64 bucket counts into the buckets' first
+// places, then places tiles of 256 groups: LDS counters rank a tile's groups
 // inside their buckets, one global atomic per bucket reserves the tile's range.
 __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb, uint32_t max_groups)
 {
     __shared__ uint32_t cnt[64];
     __shared__ uint32_t base[64];
+    __shared__ uint32_t first[64];
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;
+    if (threadIdx.x < 64) {
+        const uint32_t v = tb.sc->sched_hist[threadIdx.x];
+        first[threadIdx.x] = wave_incl_scan(v) - v;
+    }
     for (uint32_t t0 = blockIdx.x * kNT; t0 < G; t0 += gridDim.x * kNT) {
         if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
         __syncthreads();
@@ -907,7 +911,8 @@ __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb,
             local = atomicAdd(&cnt[bk], 1u);
         }
         __syncthreads();
-        if (threadIdx.x < 64 && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&mb.sched_cursor[threadIdx.x], cnt[threadIdx.x]);
+        if (threadIdx.x < 64 && cnt[threadIdx.x])
+            base[threadIdx.x] = first[threadIdx.x] + atomicAdd(&tb.sc->sched_cursor[threadIdx.x], cnt[threadIdx.x]);
         __syncthreads();
         if (g < G) mb.order[base[bk] + local] = g;
         __syncthreads();

@@ -120,14 +120,13 @@ __device__ inline float forward_wave(const float* __restrict__ P, const float* x
 // the model's weights (wb), the wavefront also evaluates the model on the
 // group's features (gcost = 2^(16 y) - 1, the predicted list length), counts
 // the group in its block's bucket histogram, and the last block scans the
-// buckets for the heaviest-first order (sched_hist_commit).
+// buckets for the heaviest-first order (sched_hist_add).
 __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
                                                         float* __restrict__ feat, uint32_t max_groups,
                                                         const uint16_t* __restrict__ wb)
 {
     __shared__ float P[kModelParams];
     __shared__ uint32_t hist[64];
-    __shared__ bool last;
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
@@ -177,7 +176,7 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
             }
         }
     }
-    if (wb) sched_hist_commit(tb, mb, hist, gridDim.x, &last);
+    if (wb) sched_hist_add(tb, hist);
 }
 
 // ------------------------------------------------------------ inference
