@@ -61,6 +61,7 @@ struct SysScalars {
     uint32_t n_heavy;
     uint32_t heavy_item; // first work item of the system's heavy direct sums
     uint32_t heavy_chunks;   // 1024-body chunks of the system (heavy direct sums)
+    uint32_t box_block;  // first block of the system in k_kick_drift_box's grid
     uint32_t heavy_slot[kMaxHeavy];   // sorted position of each heavy body, by rank
     uint32_t dcount[kMaxDepth + 1];   // boundaries splitting at each depth (mixed forms)
 };
@@ -177,11 +178,14 @@ struct TreeBuffers {
     SysScalars* sys;      // n_sys records
     const uint8_t* bsys;  // system of each body position (fixed; all 0 for one system)
     uint32_t n_sys;
+    uint32_t box_blocks;  // k_kick_drift_box's grid: blocks per system in proportion to its bodies
 };
 
 // kick (optional) + drift + per-system box; writes bb/span/side2 of every
 // system.  partials hold 6 doubles per block (at most kBoxBlocks blocks).
 constexpr unsigned kBoxBlocks = 1024;
+constexpr unsigned kBoxGrid = 512;              // the grid's usual ceiling (kMaxSystems + 256 at most)
+constexpr unsigned kBoxBodiesPerBlock = 1024;   // 4 bodies per thread
 void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
                            const TreeBuffers& tb, double* partials, hipStream_t s);
 // Morton keys (src/morton_keys.cpp:65-98) in each body's system + identity payload.

@@ -95,15 +95,25 @@ __device__ inline uint32_t lower_bound_in(const uint64_t* __restrict__ u, uint32
 }
 
 // ------------------------------------------------------ kick/drift/box
-// Block (s, k) of the grid (per_sys blocks per system) strides over system
-// s's bodies; the block that finishes last reduces every system's partial
-// boxes, one wavefront per system.  Min and max are exact, so the box does
-// not depend on the partition.
-__global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, double half_dt, int drift, double dt,
-                                                        TreeBuffers tb, uint32_t per_sys,
-                                                        double* __restrict__ partials)
+// System s owns blocks [box_block(s), box_block(s + 1)) of the grid (about
+// one per 1024 of its bodies), which stride over its bodies; the block that
+// finishes last reduces every system's partial boxes, one wavefront per
+// system.  Min and max are exact, so the box does not depend on the partition.
+__device__ inline uint32_t box_first(const TreeBuffers& tb, uint32_t s)
 {
-    const uint32_t sys = blockIdx.x / per_sys, part = blockIdx.x % per_sys;
+    return s < tb.n_sys ? tb.sys[s].box_block : tb.box_blocks;
+}
+
+__global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, double half_dt, int drift, double dt,
+                                                        TreeBuffers tb, double* __restrict__ partials)
+{
+    uint32_t sys = 0, top = tb.n_sys;   // the last system whose first block is <= blockIdx.x
+    while (top - sys > 1) {
+        const uint32_t mid = (sys + top) >> 1;
+        if (tb.sys[mid].box_block <= blockIdx.x) sys = mid;
+        else top = mid;
+    }
+    const uint32_t b0 = box_first(tb, sys), per_sys = box_first(tb, sys + 1) - b0, part = blockIdx.x - b0;
     const size_t i0 = tb.sys[sys].body_lo, i1 = tb.sys[sys].body_hi;
     double lo[3] = {INFINITY, INFINITY, INFINITY};
     double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -177,8 +187,9 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
             lo[a] = INFINITY;
             hi[a] = -INFINITY;
         }
-        for (uint32_t k = static_cast<uint32_t>(lane); k < per_sys; k += 64) {
-            const size_t blk = static_cast<size_t>(q) * per_sys + k;
+        const uint32_t q0 = box_first(tb, q), q1 = box_first(tb, q + 1);
+        for (uint32_t k = q0 + static_cast<uint32_t>(lane); k < q1; k += 64) {
+            const size_t blk = k;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 lo[a] = dmin(lo[a], __hip_atomic_load(&partials[6 * blk + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -581,14 +592,8 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
                            const TreeBuffers& tb, double* partials, hipStream_t s)
 {
-    // blocks per system: up to 512 blocks in all, at least one per system
-    unsigned g = nblk(n);
-    if (g > 512) g = 512;
-    uint32_t per_sys = g / tb.n_sys;
-    if (per_sys < 1) per_sys = 1;
-    static_assert(static_cast<unsigned>(kMaxSystems) <= kBoxBlocks, "one partial box per block");
-    k_kick_drift_box<<<per_sys * tb.n_sys, kNT, 0, s>>>(b, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, tb, per_sys,
-                                                        partials);
+    static_assert(2 * static_cast<unsigned>(kMaxSystems) <= kBoxBlocks, "one partial box per block");
+    k_kick_drift_box<<<tb.box_blocks, kNT, 0, s>>>(b, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, tb, partials);
 }
 
 void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s)
