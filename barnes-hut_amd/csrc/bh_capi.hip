@@ -174,7 +174,7 @@ int reserve(bh_ctx* c, size_t n)
     A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems);
     t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
-    A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gid, N); A(m.gfirst, N + 1); A(m.gdepth, N);
+    A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gfirst, N + 1); A(m.gdepth, N);
     A(m.g_inter, N); A(m.g_visit, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
     c->heavy_item_cap = static_cast<size_t>(bh::kMaxHeavy) * (N / bh::kHeavyChunk + bh::kMaxSystems + 1);
     A(m.hpart, 3 * c->heavy_item_cap);

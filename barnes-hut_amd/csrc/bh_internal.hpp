@@ -106,6 +106,45 @@ size_t scan_tmp_words(size_t n);
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
                         hipStream_t s);
 
+// Block-tile scans fused into their producer and consumer kernels: the
+// kernel that makes one value per thread (blocks of 256) leaves each block's
+// sum (scan_block_sum), scan_tile_offsets turns the sums into exclusive
+// offsets in place (one block; *total gets the sum), and the kernel that needs
+// the exclusive prefix adds its block's offset to the block-local scan
+// (scan_block_excl).  Every thread of the block calls the block helpers.
+// tmp holds tile_sum_words(n) u32.
+inline size_t tile_sum_words(size_t n) { return (n + 255) / 256 + 1; }
+void scan_tile_offsets(uint32_t* tsum, size_t n, uint32_t* total, hipStream_t s);
+// out[i] = exclusive prefix of in[i] (blocks of 256, offsets from scan_tile_offsets)
+void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* toff, uint32_t* out, hipStream_t s);
+// tsum[block] = sum of in over the block's 256 values
+void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s);
+
+__device__ inline void scan_block_sum(uint32_t v, uint32_t* tsum)
+{
+    __shared__ uint32_t w4[4];
+    for (int o = 32; o > 0; o >>= 1) v += static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 64));
+    if ((threadIdx.x & 63) == 0) w4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) tsum[blockIdx.x] = w4[0] + w4[1] + w4[2] + w4[3];
+}
+
+__device__ inline uint32_t scan_block_excl(uint32_t v, const uint32_t* toff)
+{
+    __shared__ uint32_t w4[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) w4[w] = inc;
+    __syncthreads();
+    uint32_t off = toff[blockIdx.x];
+    for (int q = 0; q < w; ++q) off += w4[q];
+    return off + inc - v;
+}
+
 // ----------------------------------------------------------------- sort
 // Stable LSD radix sort of (key, idx) pairs by the low `bits` key bits.
 // Input in (k0, v0); result in (k0, v0) after the call (ping-pong through
@@ -233,7 +272,6 @@ struct MixedBuffers {
     MNode* node;        // 2N
     uint32_t* slot;     // 2N: unified id -> slot
     uint32_t* base;     // N: first child slot of each branching node
-    uint32_t* gid;      // N: group start flags scanned
     uint32_t* gfirst;   // N + 1: first body of each group
     uint8_t* gdepth;    // N: tree depth of the group's cells (at the start body)
     uint32_t* g_inter;  // N: interactions evaluated by each group

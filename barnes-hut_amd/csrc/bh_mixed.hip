@@ -223,36 +223,43 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     }
 }
 
-__device__ inline void group_first(size_t i, size_t n, const TreeBuffers& tb, const MixedBuffers& mb)
+__device__ inline void group_first(size_t i, size_t n, uint32_t gid, const TreeBuffers& tb, const MixedBuffers& mb)
 {
-    if (tb.gflag[i]) mb.gfirst[mb.gid[i]] = static_cast<uint32_t>(i);
+    if (tb.gflag[i]) mb.gfirst[gid] = static_cast<uint32_t>(i);
     if (i == 0) mb.gfirst[tb.sc->n_groups] = static_cast<uint32_t>(n);
 }
 
 // number of children of every branching node (into base, scanned next)
-__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, uint32_t* __restrict__ out)
+__device__ inline uint32_t child_count(const TreeBuffers& tb, size_t bi)
 {
-    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (bi >= n) return;
     uint32_t c = 0;
     if (bi < tb.sc->n_branch) {
         const int32_t* ch = tb.child + 8 * bi;
 #pragma unroll
         for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
     }
-    out[bi] = c;
+    return c;
+}
+
+// block sums of the child counts (k_slots scans them)
+__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    scan_block_sum(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp);
 }
 
 // slots: the root of system s at s; children of branching node b at
-// n_sys + base[b] + rank
+// n_sys + base[b] + rank, base = the exclusive scan of the child counts
 __global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb)
 {
     const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t base = scan_block_excl(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp);
+    if (bi < n) mb.base[bi] = base;
     if (bi < tb.n_sys) mb.slot[tb.sys[bi].root] = static_cast<uint32_t>(bi);
     if (bi >= tb.sc->n_branch) return;
     const int32_t* ch = tb.child + 8 * bi;
     uint32_t k = 0;
-    const uint32_t b0 = tb.n_sys + mb.base[bi];
+    const uint32_t b0 = tb.n_sys + base;
 #pragma unroll
     for (int o = 0; o < 8; ++o)
         if (ch[o] >= 0) mb.slot[ch[o]] = b0 + k++;
@@ -262,7 +269,10 @@ __global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBu
 __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb, double G)
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (t < n) group_first(t, n, tb, mb);
+    if (static_cast<size_t>(blockIdx.x) * kNT < n) {   // block-uniform: the blocks over the bodies
+        const uint32_t gid = scan_block_excl(t < n ? tb.gflag[t] : 0u, tb.scan_tmp);
+        if (t < n) group_first(t, n, gid, tb, mb);
+    }
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
     if (t >= static_cast<size_t>(M) + B) return;
     const double* q = tb.lraw + 4 * t;
@@ -981,8 +991,8 @@ void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const Grou
 {
     k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb);
     k_sys_extent<<<1, kNT, 0, s>>>(tb, gp);
-    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, mb.order);
-    scan_exclusive_u32(mb.order, mb.base, nullptr, n, tb.scan_tmp, s);
+    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb);
+    scan_tile_offsets(tb.scan_tmp, n, nullptr, s);
     k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
 }
 
@@ -990,7 +1000,10 @@ void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffer
                          double G, hipStream_t s)
 {
     k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
-    scan_exclusive_u32(tb.gflag, mb.gid, &tb.sc->n_groups, n, tb.scan_tmp, s);
+    // group index of every first body: the group flags scanned, the block
+    // sums here, the block-local scan in k_mixed_nodes
+    scan_tile_sums256(tb.gflag, n, tb.scan_tmp, s);
+    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_groups, s);
     k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, b, tb, mb, G);
 }
 

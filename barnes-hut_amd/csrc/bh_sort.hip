@@ -30,6 +30,8 @@
 // order (system, key) and keeps every system inside its own index range.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bh_internal.hpp"
 
 namespace bh {
@@ -211,6 +213,21 @@ __global__ __launch_bounds__(kNT) void k_scan_sums(uint32_t* __restrict__ sums, 
         carry += total;
     }
     if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(256) void k_tile_sums256(const uint32_t* __restrict__ in, size_t n,
+                                                      uint32_t* __restrict__ tsum)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    scan_block_sum(i < n ? in[i] : 0u, tsum);
+}
+
+__global__ __launch_bounds__(256) void k_tile_apply256(const uint32_t* __restrict__ in, size_t n,
+                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ out)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+    const uint32_t e = scan_block_excl(i < n ? in[i] : 0u, toff);
+    if (i < n) out[i] = e;
 }
 
 // in and out may alias: a tile reads its values before it writes them
@@ -694,7 +711,22 @@ __global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uin
 
 }  // namespace
 
-size_t scan_tmp_words(size_t cap) { return scan_tiles(cap) + 1; }
+size_t scan_tmp_words(size_t cap) { return std::max(scan_tiles(cap) + 1, tile_sum_words(cap)); }
+
+void scan_tile_offsets(uint32_t* tsum, size_t n, uint32_t* total, hipStream_t s)
+{
+    k_scan_sums<<<1, kNT, 0, s>>>(tsum, (n + 255) / 256, total);
+}
+
+void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* toff, uint32_t* out, hipStream_t s)
+{
+    k_tile_apply256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, toff, out);
+}
+
+void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s)
+{
+    k_tile_sums256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, tsum);
+}
 
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,
                         hipStream_t s)

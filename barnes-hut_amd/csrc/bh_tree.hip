@@ -267,10 +267,8 @@ __global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, TreeBuffers tb
 }
 
 // ---------------------------------------------------------- permute
-__global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, TreeBuffers tb, size_t n)
+__device__ inline uint32_t permute_one(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t i)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (i >= n) return;
     const uint32_t j = tb.perm[i];
     const uint8_t sys = tb.bsys[i];   // = bsys[j]: sorting stays inside a system
     const int8_t hr = src.hrank[j];
@@ -285,18 +283,31 @@ __global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, TreeBuf
     dst.vz[i] = src.vz[j];
     dst.m[i] = src.m[j];
     dst.id[i] = src.id[j];
-    tb.flag[i] = (i == 0 || tb.key[i] != tb.key[i - 1] || tb.bsys[i - 1] != sys) ? 1u : 0u;
+    const uint32_t f = (i == 0 || tb.key[i] != tb.key[i - 1] || tb.bsys[i - 1] != sys) ? 1u : 0u;
+    tb.flag[i] = f;
+    return f;
+}
+
+// Also the first-of-key flags, and their block sums for the leaf scan.
+__global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, TreeBuffers tb, size_t n)
+{
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    uint32_t f = 0;
+    if (i < n) f = permute_one(src, dst, tb, i);
+    scan_block_sum(f, tb.scan_tmp);
 }
 
 // ------------------------------------------------------------- tree
 // leaves: distinct keys, first body of each leaf, leaf id of each body
+// (the leaf index: the scan of the flags, offsets from scan_tile_offsets)
 __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key, const uint32_t* __restrict__ flag,
-                                                const uint32_t* __restrict__ excl, size_t n, TreeBuffers tb)
+                                                size_t n, TreeBuffers tb)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t f = i < n ? flag[i] : 0u;
+    const uint32_t ex = scan_block_excl(f, tb.scan_tmp);
     if (i >= n) return;
-    const uint32_t f = flag[i];
-    const uint32_t j = excl[i] + f - 1;
+    const uint32_t j = ex + f - 1;
     tb.leaf_of[i] = j;
     if (tb.gflag) tb.gflag[i] = 0;
     if (f) {
@@ -310,10 +321,8 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
 
 // leaf moments (raw, or light moments and boxes for the mixed forms);
 // boundary depths and representatives
-__global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n, TreeBuffers tb)
+__device__ inline uint32_t leaf_moments_bounds(const Bodies& b, const TreeBuffers& tb, size_t j)
 {
-    const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (j >= n) return;
     const uint32_t M = tb.sc->n_leaves;
     tb.cnt_hi[j] = 0;
     if (j < M) {
@@ -383,6 +392,16 @@ __global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n,
         }
     }
     tb.is_rep[j] = isrep;
+    return isrep;
+}
+
+// (and the block sums of the representative flags for the branch scan)
+__global__ __launch_bounds__(kNT) void k_leaf_moments_bounds(Bodies b, size_t n, TreeBuffers tb)
+{
+    const size_t j = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    uint32_t isrep = 0;
+    if (j < n) isrep = leaf_moments_bounds(b, tb, j);
+    scan_block_sum(isrep, tb.scan_tmp);
 }
 
 // branching node records; boundary -> branching index
@@ -614,10 +633,13 @@ void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s)
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s)
 {
     // leaf index of every body: exclusive scan of first-of-key flags (M total)
-    scan_exclusive_u32(tb.flag, tb.leaf_of, &tb.sc->n_leaves, n, tb.scan_tmp, s);
-    k_leaves<<<nblk(n), kNT, 0, s>>>(tb.key, tb.flag, tb.leaf_of, n, tb);
+    // leaf index of every body: the first-of-key flags scanned (k_permute
+    // left the block sums); M = n_leaves
+    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_leaves, s);
+    k_leaves<<<nblk(n), kNT, 0, s>>>(tb.key, tb.flag, n, tb);
     k_leaf_moments_bounds<<<nblk(n), kNT, 0, s>>>(b, n, tb);
-    scan_exclusive_u32(tb.is_rep, tb.bidx, &tb.sc->n_branch, n, tb.scan_tmp, s);
+    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_branch, s);
+    scan_tile_apply(tb.is_rep, n, tb.scan_tmp, tb.bidx, s);
     k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
     k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
     if (moments) launch_tree_moments(n, tb, s);
