@@ -19,7 +19,8 @@
 // launches (no look-back: at these sizes the chain's latency costs more
 // than reading the values twice).
 //
-// Sort: 8-bit digits.  One launch counts the digits of every pass (per-block
+// Radix sort (the full sort of bh_set_sort_mode(BH_SORT_FULL)): 8-bit
+// digits.  One launch counts the digits of every pass (per-block
 // LDS histograms, reduced and scanned by the last block), then one launch
 // per pass: every wavefront ranks its 512 keys with ballots (the peers of a
 // lane are the lanes that share its digit) and a running per-wave digit count
@@ -458,16 +459,18 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
 // order against its kNsWin neighbours on either side (bodies of different
 // systems are always in order: a system owns a fixed index range); if the
 // kept keys, in index order, are then sorted as a whole, only the removed
-// ones go through the radix sort, and a merge of the two sorted runs gives
-// the unique stable order -- equal keys by input position -- which the full
-// radix sort would give.  If the check fails, or more than half of the keys
-// were removed, every key goes through the radix sort (full mode).
+// ones are sorted (a merge sort: bitonic tiles, then merge rounds), and a
+// merge of the two sorted runs gives the unique stable order -- equal keys by
+// input position -- which the full radix sort gives.  If the check fails, or
+// more than half of the keys were removed, the merge sort takes every key
+// (full mode).
 //
-// Control words (ctl): [0] mode (1 near, 0 full), [1] count of the radix
-// sort (removed keys, or n), [2] kept keys (0 in full mode), [3] keys of the
-// sorted run the merge takes (= [1]), [5] removed keys.  (A one-workgroup LDS
-// sort of up to 12288 removed keys measured 94 us at 300k bodies -- one CU
-// doing eight passes -- against 91 us for the multi-block passes.)
+// Control words (ctl): [0] mode (1 near, 0 full), [1] count of the merge sort
+// (removed keys, or n), [2] kept keys (0 in full mode), [3] keys of the sorted
+// run the last merge takes (= [1]), [5] removed keys.  (Measured at 300k
+// bodies, 11.6k removed keys: the radix sort of the removed keys -- a
+// histogram and eight look-back passes of about 11 us each -- 106 us; one
+// workgroup's LDS radix sort of them 94 us; the merge sort 58 us.)
 constexpr int kNsWin = 4;
 constexpr int kNsIPT = 8;
 constexpr int kNsTile = kNT * kNsIPT;   // 2048 keys per tile
@@ -682,17 +685,133 @@ __device__ inline uint32_t ns_rank(const uint64_t* ks, const uint32_t* vs, uint3
     return lo;
 }
 
+// ---- merge sort of the removed keys (near mode) or of every key (full mode)
+// Tiles of kMsTile keys are sorted in LDS by a bitonic network, then rounds
+// of pairwise merges double the sorted runs: an element's place in the merged
+// run is its index in its own run plus its rank (binary search) in the other.
+// The order is (system, key, input position), a total order, so the result
+// is the unique stable order.  Count m = ctl[1]; a round whose run length
+// reaches m has nothing to do, so rounds(m) rounds write and the result ends
+// in (k1, v1) after an even number of them, in (k2, v2) after an odd one.
+constexpr uint32_t kMsTile = 2048;
+constexpr int kMsNT = 1024;   // threads of the tile sort: one compare-exchange each per phase
+
+__device__ inline uint32_t merge_rounds(uint32_t m)
+{
+    uint32_t r = 0;
+    for (uint64_t L = kMsTile; L < m; L <<= 1) ++r;
+    return r;
+}
+
+// Tile sort: input (k2, v2) in near mode, (k0, v0) in full mode; output (k1, v1).
+// A bitonic network in LDS, 66 phases.  (Measured: 256 threads with four
+// exchanges each take 80 us at 300k bodies, these 1024 threads 35 us; 8192-key
+// tiles held in registers, shuffles for the near partners, take 144 us: the
+// few tiles of a small count leave the work to a few CUs.)
+__global__ __launch_bounds__(kMsNT) void k_ns_tile_sort(const uint64_t* __restrict__ k0, const uint32_t* __restrict__ v0,
+                                                        const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
+                                                        uint64_t* __restrict__ k1, uint32_t* __restrict__ v1,
+                                                        const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
+{
+    __shared__ uint64_t sk[kMsTile];
+    __shared__ uint32_t sv[kMsTile];
+    __shared__ uint8_t ss[kMsTile];
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t m = nt.ctl[1];
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * kMsTile;
+    if (t0 >= m) return;
+    const uint64_t* ik = nt.ctl[0] == 1u ? k2 : k0;
+    const uint32_t* iv = nt.ctl[0] == 1u ? v2 : v0;
+    for (uint32_t e = threadIdx.x; e < kMsTile; e += kMsNT) {
+        const size_t i = t0 + e;
+        if (i < m) {
+            sk[e] = ik[i];
+            sv[e] = iv[i];
+            ss[e] = sysd ? sysd[sv[e]] : 0;
+        } else {   // padding above every real entry
+            sk[e] = ~0ull;
+            sv[e] = ~0u;
+            ss[e] = 0xff;
+        }
+    }
+    __syncthreads();
+    const uint32_t p = threadIdx.x;   // kMsTile / 2 == kMsNT pairs
+    for (uint32_t k = 2; k <= kMsTile; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t lj = 31u - static_cast<uint32_t>(__builtin_clz(j));   // j = 2^lj
+            const uint32_t a = ((p >> lj) << (lj + 1)) | (p & (j - 1)), b = a + j;
+            const bool up = (a & k) == 0;
+            const bool swap = up ? ns_less(ss[b], sk[b], sv[b], ss[a], sk[a], sv[a])
+                                 : ns_less(ss[a], sk[a], sv[a], ss[b], sk[b], sv[b]);
+            if (swap) {
+                const uint64_t tk = sk[a];
+                sk[a] = sk[b];
+                sk[b] = tk;
+                const uint32_t tv = sv[a];
+                sv[a] = sv[b];
+                sv[b] = tv;
+                const uint8_t ts = ss[a];
+                ss[a] = ss[b];
+                ss[b] = ts;
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t e = threadIdx.x; e < kMsTile; e += kMsNT) {
+        const size_t i = t0 + e;
+        if (i < m) {
+            k1[i] = sk[e];
+            v1[i] = sv[e];
+        }
+    }
+}
+
+// One round: runs of length L in (ik, iv) merged pairwise into (ok, ov).
+__global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
+                                                        uint64_t* __restrict__ ok, uint32_t* __restrict__ ov,
+                                                        const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap,
+                                                        uint32_t L)
+{
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t m = nt.ctl[1];
+    if (L >= m) return;
+    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (i >= m) return;
+    const size_t start = i / (2 * static_cast<size_t>(L)) * (2 * static_cast<size_t>(L));
+    const size_t a1 = min(start + L, static_cast<size_t>(m)), b1 = min(start + 2 * static_cast<size_t>(L), static_cast<size_t>(m));
+    const uint64_t k = ik[i];
+    const uint32_t v = iv[i];
+    const uint8_t s = sysd ? sysd[v] : 0;
+    size_t pos;
+    if (i < a1) {
+        if (a1 >= b1) {   // no partner run
+            ok[i] = k;
+            ov[i] = v;
+            return;
+        }
+        pos = i + ns_rank(ik + a1, iv + a1, static_cast<uint32_t>(b1 - a1), sysd, s, k, v);
+    } else {
+        pos = start + (i - a1) + ns_rank(ik + start, iv + start, static_cast<uint32_t>(a1 - start), sysd, s, k, v);
+    }
+    ok[pos] = k;
+    ov[pos] = v;
+}
+
 // Merge of the kept run ((k1, v1) + R, K keys) and the sorted removed run
-// (fk, fv: R keys) into (k0, v0).  Full mode: K = 0, a copy.
+// (R keys, in (k1, v1) or (k2, v2) by merge_rounds(R)) into (k0, v0).  Full
+// mode: K = 0, a copy.
 __global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
                                                   const uint64_t* __restrict__ k1, const uint32_t* __restrict__ v1,
-                                                  const uint64_t* __restrict__ fk, const uint32_t* __restrict__ fv,
+                                                  const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
                                                   const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (i >= n) return;
     const NsTmp nt = ns_tmp_at(tmp, cap);
     const uint32_t R = nt.ctl[3], K = nt.ctl[2];
+    const bool odd = merge_rounds(nt.ctl[1]) & 1u;
+    const uint64_t* fk = odd ? k2 : k1;
+    const uint32_t* fv = odd ? v2 : v1;
     uint64_t k;
     uint32_t v, pos;
     if (i < K) {
@@ -752,29 +871,21 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     const unsigned T = static_cast<unsigned>(ns_tiles(n));
     k_ns_mark<<<T, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
     k_ns_scatter<<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
-    const uint32_t* ctl = ns_tmp_at(tmp, cap).ctl;
-    const int key_passes = (bits + 7) / 8;   // <= 8
-    const int passes = key_passes + (sysd ? 1 : 0);
-    const bool big = n >= kBigSort;
-    const unsigned nt = static_cast<unsigned>(sort_tiles(n, big ? 2 * kSortTile : kSortTile));
-    // pass p writes (k1, v1) for even p, (k2, v2) for odd p; pass 0 reads
-    // (k0, v0) in full mode and the removed keys (k2, v2) in near mode
-    if (big) k_sort_hist_all<16><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, k2, v2, ctl);
-    else k_sort_hist_all<8><<<nt, kNT, 0, s>>>(k0, v0, n, key_passes, sysd, tmp, k2, v2, ctl);
-    for (int p = 0; p < passes; ++p) {
-        const uint8_t* sd = p < key_passes ? nullptr : sysd;
-        const uint64_t* ka = p == 0 ? k0 : (p % 2 ? k1 : k2);
-        const uint32_t* va = p == 0 ? v0 : (p % 2 ? v1 : v2);
-        uint64_t* kb = p % 2 ? k2 : k1;
-        uint32_t* vb = p % 2 ? v2 : v1;
-        const uint64_t* kn = p == 0 ? k2 : nullptr;
-        const uint32_t* vn = p == 0 ? v2 : nullptr;
-        if (big) k_sort_pass<16><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, kn, vn, ctl);
-        else k_sort_pass<8><<<nt, kNT, 0, s>>>(ka, va, kb, vb, n, p, sd, tmp, kn, vn, ctl);
+    (void)bits;   // the merge sort compares whole keys
+    // tile sort into (k1, v1), then rounds ping-ponging k1 -> k2 -> k1 ...
+    k_ns_tile_sort<<<static_cast<unsigned>((n + kMsTile - 1) / kMsTile), kMsNT, 0, s>>>(k0, v0, k2, v2, k1, v1, sysd,
+                                                                                     tmp, cap);
+    uint64_t* ka = k1;
+    uint32_t* va = v1;
+    uint64_t* kb = k2;
+    uint32_t* vb = v2;
+    for (size_t L = kMsTile; L < n; L <<= 1) {
+        k_ns_merge_round<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(ka, va, kb, vb, sysd, tmp, cap,
+                                                                                  static_cast<uint32_t>(L));
+        std::swap(ka, kb);
+        std::swap(va, vb);
     }
-    const uint64_t* fk = (passes - 1) % 2 ? k2 : k1;
-    const uint32_t* fv = (passes - 1) % 2 ? v2 : v1;
-    k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, fk, fv, sysd, tmp, cap);
+    k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, sysd, tmp, cap);
 }
 
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
