@@ -766,17 +766,11 @@ __global__ __launch_bounds__(kMsNT) void k_ns_tile_sort(const uint64_t* __restri
     }
 }
 
-// One round: runs of length L in (ik, iv) merged pairwise into (ok, ov).
-__global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
-                                                        uint64_t* __restrict__ ok, uint32_t* __restrict__ ov,
-                                                        const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap,
-                                                        uint32_t L)
+// element i of a merge round (runs of length L, count m)
+__device__ inline void merge_one(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
+                                 uint64_t* __restrict__ ok, uint32_t* __restrict__ ov, const uint8_t* __restrict__ sysd,
+                                 uint32_t m, uint32_t L, size_t i)
 {
-    const NsTmp nt = ns_tmp_at(tmp, cap);
-    const uint32_t m = nt.ctl[1];
-    if (L >= m) return;
-    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (i >= m) return;
     const size_t start = i / (2 * static_cast<size_t>(L)) * (2 * static_cast<size_t>(L));
     const size_t a1 = min(start + L, static_cast<size_t>(m)), b1 = min(start + 2 * static_cast<size_t>(L), static_cast<size_t>(m));
     const uint64_t k = ik[i];
@@ -796,6 +790,20 @@ __global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restri
     ok[pos] = k;
     ov[pos] = v;
 }
+
+// One round: runs of length L in (ik, iv) merged pairwise into (ok, ov).
+__global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
+                                                        uint64_t* __restrict__ ok, uint32_t* __restrict__ ov,
+                                                        const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap,
+                                                        uint32_t L)
+{
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t m = nt.ctl[1];
+    if (L >= m) return;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x; i < m; i += static_cast<size_t>(gridDim.x) * kNT)
+        merge_one(ik, iv, ok, ov, sysd, m, L, i);
+}
+
 
 // Merge of the kept run ((k1, v1) + R, K keys) and the sorted removed run
 // (R keys, in (k1, v1) or (k2, v2) by merge_rounds(R)) into (k0, v0).  Full
