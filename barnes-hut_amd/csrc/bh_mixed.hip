@@ -30,7 +30,8 @@
 // z - c, G m) with c the group centre (FP64), so the offsets keep FP32
 // precision near the group.  A full list is evaluated by all 64 lanes:
 // FP32 FMA over blocks of 32 entries, each block sum added into a
-// Kahan-compensated FP32 accumulator.  A group's own bodies appear in the
+// Kahan-compensated FP32 accumulator; lanes past a group's bodies take over
+// stretches of the bodies' lists.  A group's own bodies appear in the
 // list with offsets bit-identical to the lane's own, so r^2 = 0 marks (and
 // drops) the self term.
 //
@@ -443,14 +444,106 @@ __device__ inline void flush_list_k(const float4* __restrict__ list, uint32_t ln
     }
 }
 
+// p[0..R) (R wave-uniform) for one body: FP32 FMA inside blocks of 32
+// entries, each block sum added into a; the last, partial block loads eight
+// entries at a time as well.
+__device__ inline void seg_sum(const float4* __restrict__ p, uint32_t R, float ox, float oy, float oz, float soft2,
+                               Acc3& a)
+{
+    uint32_t j = 0;
+    for (; j + 32u <= R; j += 32u) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+        for (int u = 0; u < 32; u += 8) {
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = p[j + static_cast<uint32_t>(u + k)];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) interact(q[k], ox, oy, oz, soft2, bx, by, bz);
+        }
+        a.add(bx, by, bz);
+    }
+    if (j < R) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        for (; j + 8u <= R; j += 8u) {
+            float4 q[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = p[j + static_cast<uint32_t>(k)];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) interact(q[k], ox, oy, oz, soft2, bx, by, bz);
+        }
+        for (; j < R; ++j) interact(p[j], ox, oy, oz, soft2, bx, by, bz);
+        a.add(bx, by, bz);
+    }
+}
+
+// Shortest share of a list, in entries, that the idle lanes of a group take
+// over: shorter shares cost more in hand-over than they save (measured on the
+// 300k walk: 16 entries +2.5%, 24 -0.5%, 32 -2.5%, 48 -2%).
+constexpr uint32_t kMinShare = 32;
+
+// One lane per body.  In a group of cnt < 64 bodies the E = 64 - cnt idle
+// lanes help: with q = ceil(cnt / E), R = floor(ln / (q + 1)) rounded down to
+// a multiple of 8 (stretches load 8 entries at a time) and T = q R, the
+// body lanes take the entries [0, T) of their body in q stretches of R, while
+// idle lane e takes [T, T + R) of bodies e, e + E, e + 2E, ... in the same
+// stretches; the body lanes then take [T + R, ln).  A helper's stretch is a
+// compensated sum of its own, handed to the body's lane through hp (LDS,
+// 3 x 64 doubles per wave) and added in FP64 into (rx, ry, rz), in list
+// order.  Every body's sum depends on its group and list alone.  All passes
+// share one loop body, which keeps the kernel's code within the instruction
+// cache.
+__device__ inline void flush_list_1(const float4* __restrict__ list, uint32_t ln, uint32_t cnt, float ox, float oy,
+                                    float oz, float soft2, Acc3& a, double (*hp)[64], double& rx, double& ry,
+                                    double& rz)
+{
+    ln = uni(ln);
+    const uint32_t lane = __lane_id();
+    const uint32_t E = 64u - cnt;
+    uint32_t q = 0, R = 0;
+    if (E > 0u) {
+        q = (cnt + E - 1u) / E;
+        R = (ln / (q + 1u)) & ~7u;
+        if (R < kMinShare) q = 0;   // no helpers: one pass over the list
+    }
+    const uint32_t T = q * R;
+    const bool helper = lane >= cnt;
+    for (uint32_t s = 0; s <= q; ++s) {
+        const bool last = s == q;   // the body lanes' rest of the list
+        const uint32_t hb = lane - cnt + s * E;   // the helper's body in stretch s
+        const bool hv = helper && !last && hb < cnt;
+        const int src = static_cast<int>(hv ? hb : lane);
+        const float px = __shfl(ox, src, 64), py = __shfl(oy, src, 64), pz = __shfl(oz, src, 64);
+        if (helper && !last) a = Acc3();
+        const uint32_t from = last ? T + (q ? R : 0u) : (helper ? T : s * R);
+        seg_sum(list + from, last ? ln - T - (q ? R : 0u) : R, px, py, pz, soft2, a);
+        if (hv) {
+            hp[0][hb] = static_cast<double>(a.sx) - static_cast<double>(a.cx);
+            hp[1][hb] = static_cast<double>(a.sy) - static_cast<double>(a.cy);
+            hp[2][hb] = static_cast<double>(a.sz) - static_cast<double>(a.cz);
+        }
+    }
+    if (q) {
+        __builtin_amdgcn_wave_barrier();
+        if (!helper) {
+            rx += hp[0][lane];
+            ry += hp[1][lane];
+            rz += hp[2][lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // K (wave-uniform) = lanes per body: 4 for groups of <= 16 bodies, 2 for
-// <= 32, else 1, so small groups keep every lane busy.
-__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub, float ox,
-                                  float oy, float oz, float soft2, Acc3& a)
+// <= 32, else 1 (then the lanes past the group's bodies help), so small
+// groups keep every lane busy.
+__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub, uint32_t cnt,
+                                  float ox, float oy, float oz, float soft2, Acc3& a, double (*hp)[64], double& rx,
+                                  double& ry, double& rz)
 {
     if (K == 4) flush_list_k<4>(list, ln, sub, ox, oy, oz, soft2, a);
     else if (K == 2) flush_list_k<2>(list, ln, sub, ox, oy, oz, soft2, a);
-    else flush_list_k<1>(list, ln, sub, ox, oy, oz, soft2, a);
+    else flush_list_1(list, ln, cnt, ox, oy, oz, soft2, a, hp, rx, ry, rz);
 }
 
 constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
@@ -529,16 +622,19 @@ __device__ inline void heavy_item(const Bodies& b, const TreeBuffers& tb, const 
     __builtin_amdgcn_wave_barrier();
 }
 
-// 151 VGPRs: 3 waves per SIMD.  Capping at 128 for 4 waves measured 1% slower
-// on the 300k walk and 4% faster at 2M (the list loop needs its registers).
-__global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
+// 157 VGPRs: 3 waves per SIMD (held there by amdgpu_waves_per_eu).  Capping
+// at 128 for 4 waves measured 1% slower on the 300k walk and 4% faster at 2M
+// (the list loop needs its registers).
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_mixed(Bodies b, size_t n, TreeBuffers tb, MixedBuffers mb,
                                                     MixedParams p)
 {
     __shared__ float4 s_list[kWaves][kListCap];
     __shared__ uint32_t s_stack[kWaves][kStackCap];
     __shared__ uint32_t s_nodes[kWaves][kNodeBuf];
+    __shared__ double s_help[kWaves][3][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float4* list = s_list[w];
+    double(*hp)[64] = s_help[w];
     uint32_t* stk = s_stack[w];
     uint32_t* nbuf = s_nodes[w];
     const uint32_t G = tb.sc->n_groups;
@@ -618,6 +714,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
         const float oy = static_cast<float>(y - cy);
         const float oz = static_cast<float>(z - cz);
         Acc3 acc;
+        double rx = 0.0, ry = 0.0, rz = 0.0;   // the helpers' stretches (one lane per body)
         uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0;
         bool overflow = false;
         const uint32_t gsys = tb.bsys[b0];
@@ -734,7 +831,7 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
                 const uint32_t na = static_cast<uint32_t>(__popcll(am));
                 if (ln + na > static_cast<uint32_t>(kListCap)) {
                     __builtin_amdgcn_wave_barrier();
-                    flush_list(list, ln, K, sub, ox, oy, oz, soft2f, acc);
+                    flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
                     __builtin_amdgcn_wave_barrier();
                     ln = 0;
                 }
@@ -770,12 +867,13 @@ __global__ __launch_bounds__(kNT) void k_walk_mixed(Bodies b, size_t n, TreeBuff
             }
         }
         __builtin_amdgcn_wave_barrier();
-        flush_list(list, ln, K, sub, ox, oy, oz, soft2f, acc);
+        flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
         __builtin_amdgcn_wave_barrier();
-        // the K partial sums of a body, added in FP64 in sub order
-        double lax = static_cast<double>(acc.sx) + static_cast<double>(acc.cx);
-        double lay = static_cast<double>(acc.sy) + static_cast<double>(acc.cy);
-        double laz = static_cast<double>(acc.sz) + static_cast<double>(acc.cz);
+        // the K partial sums of a body, added in FP64 in sub order (a Kahan
+        // sum's value is s - c: c holds the negative of the rounding lost)
+        double lax = static_cast<double>(acc.sx) - static_cast<double>(acc.cx) + rx;
+        double lay = static_cast<double>(acc.sy) - static_cast<double>(acc.cy) + ry;
+        double laz = static_cast<double>(acc.sz) - static_cast<double>(acc.cz) + rz;
         if (K > 1) {
             double tx = 0.0, ty = 0.0, tz = 0.0;
             for (uint32_t k = 0; k < K; ++k) {
