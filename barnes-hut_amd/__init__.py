@@ -84,6 +84,7 @@ SIGNATURES = [
     ("bh_get_groups", C.c_int, [_P, C.POINTER(_I64), _I64, _P, _P, _P]),
     ("bh_set_schedule", C.c_int, [_P, C.c_int]),
     ("bh_get_group_timeline", C.c_int, [_P, _I64, _P, _P, _P]),
+    ("bh_get_group_work", C.c_int, [_P, _I64, _P]),
     ("bh_model_configure", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_float]),
     ("bh_model_reset", C.c_int, [_P, C.c_uint64]),
     ("bh_model_get_params", C.c_int, [_P, _P]),
@@ -434,6 +435,14 @@ class Engine:
         w = np.zeros(k.value, np.uint32)
         self._ck(self.lib.bh_get_group_timeline(self.h, k.value, _ptr(t0), _ptr(t1), _ptr(w)))
         return t0, t1, w
+
+    def group_work(self):
+        """Walk work per group of the last mixed walk: list steps + visits / 2."""
+        k = _I64(0)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
+        work = np.zeros(k.value, np.uint32)
+        self._ck(self.lib.bh_get_group_work(self.h, k.value, _ptr(work)))
+        return work
 
     def groups(self):
         """Walk groups of the last mixed evaluation: first_body (G+1),

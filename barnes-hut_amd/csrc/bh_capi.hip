@@ -77,7 +77,7 @@ struct bh_ctx {
     bool have_tree = false;
     bool have_counts = false;
     bool have_groups = false;
-    bool prev_counts = false;         // the bodies carry list lengths of an earlier mixed walk
+    bool prev_counts = false;         // the bodies carry the walk work of an earlier mixed walk
     hipEvent_t ev[5] = {};
     // per-step phase events of bh_step: ring of kRing sets of 5
     std::vector<hipEvent_t> ring;
@@ -175,7 +175,7 @@ int reserve(bh_ctx* c, size_t n)
     t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gfirst, N + 1); A(m.gdepth, N);
-    A(m.g_inter, N); A(m.g_visit, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
+    A(m.g_inter, N); A(m.g_visit, N); A(m.g_work, N); A(m.g_t0, N); A(m.g_t1, N); A(m.g_worker, N); A(m.order, N); A(m.chunk, 65536); A(m.gcost, N);
     c->heavy_item_cap = static_cast<size_t>(bh::kMaxHeavy) * (N / bh::kHeavyChunk + bh::kMaxSystems + 1);
     A(m.hpart, 3 * c->heavy_item_cap);
     A(c->md.feat, static_cast<size_t>(bh::kModelFeatures) * N);
@@ -284,9 +284,9 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
         bh::launch_walk_mixed(c->cur, c->n, tb, c->mb, mw, c->mixed_blocks, s);
         bh::launch_heavy_direct(c->cur, tb, c->mb, mp, heavy_in_walk, s);
         if (training) {
-            // training phase: the walk's list lengths become samples, then
+            // training phase: the walk's recorded work becomes samples, then
             // iters_per_step Adam updates on mini-batches of all samples so far.
-            // The first walk of a body set has no previous list lengths
+            // The first walk of a body set has no previous work
             // (feature 5 is 0 for every group): its groups are no samples.
             if (c->prev_counts) {
                 bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
@@ -1137,6 +1137,19 @@ int bh_get_group_timeline(bh_ctx* c, int64_t capacity, uint64_t* t_start, uint64
     return BH_OK;
 }
 
+int bh_get_group_work(bh_ctx* c, int64_t capacity, uint32_t* work)
+{
+    if (check_ctx(c) || !work) return BH_ERR_ARG;
+    if (!c->have_groups) return fail(c, BH_ERR_STATE, "no mixed evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    const size_t G = sc.n_groups;
+    if (capacity < static_cast<int64_t>(G)) return fail(c, BH_ERR_OVERFLOW, "%zu groups", G);
+    HIP_TRY(c, hipMemcpy(work, c->mb.g_work, G * 4, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
 int bh_direct_sum(bh_ctx* c, int64_t m, const int64_t* idx, double G, double soft2, bh_vec3* acc_out)
 {
     if (check_ctx(c)) return BH_ERR_ARG;
@@ -1202,7 +1215,7 @@ int bh_model_reset(bh_ctx* c, uint64_t seed)
     if (check_ctx(c)) return BH_ERR_ARG;
     // He-uniform weights for the ReLU layers, a zero output layer and zero
     // biases: the untrained network returns its fallback (feature 5, the
-    // previous step's list length) exactly, and training moves it only where
+    // previous step's walk work) exactly, and training moves it only where
     // the loss falls
     std::vector<float> w(bh::kModelParams, 0.f);
     uint64_t z = seed;

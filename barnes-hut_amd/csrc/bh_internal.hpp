@@ -80,8 +80,8 @@ struct StepScalars {
     uint32_t cost_total;     // sum of the group weights of the split schedule
     uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
     uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
-    uint32_t sched_hist[64];    // groups per cost bucket of the heaviest-first order
-    uint32_t sched_cursor[64];  // groups placed per bucket so far (both cleared by k_kick_drift_box)
+    uint32_t sched_hist[128];   // groups per cost bucket of the heaviest-first order
+    uint32_t sched_cursor[128]; // groups placed per bucket so far (both cleared by k_kick_drift_box)
 };
 
 // In key order the mixed walk deals its groups out in runs of kWalkRun
@@ -173,7 +173,7 @@ struct Bodies {
     double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *m;
     uint64_t* id;
     int8_t* hrank;   // rank in the heavy (direct FP64) set, -1 for light bodies
-    uint32_t* pcost; // list length the body's group evaluated in the last mixed walk
+    uint32_t* pcost; // walk work of the body's group in the last mixed walk (list steps + visits / 2)
 };
 
 struct TreeBuffers {
@@ -276,6 +276,7 @@ struct MixedBuffers {
     uint8_t* gdepth;    // N: tree depth of the group's cells (at the start body)
     uint32_t* g_inter;  // N: interactions evaluated by each group
     uint32_t* g_visit;  // N: nodes visited by each group
+    uint32_t* g_work;   // N: walk work of each group: list-evaluation steps + visits / 2
     uint64_t* g_t0;     // N: wall clock (100 MHz) when a worker took the group
     uint64_t* g_t1;     // N: wall clock when it finished
     uint32_t* g_worker; // N: worker wave that walked the group
@@ -286,21 +287,28 @@ struct MixedBuffers {
 };
 
 // ---------------------------------------------------- schedule histogram
-// Heaviest-first order: 64 buckets of the estimated list length c, two per
-// octave, bucket 0 the costliest.
+// Heaviest-first order: 128 buckets of the estimated walk work c (list steps
+// + visits / 2), eight per octave, bucket 0 the costliest.  (Simulated with
+// the measured group times of the 300k walk: two buckets per octave of the
+// list length leave the last wave 9% later than eight per octave of the
+// work.)
+constexpr int kSchedBuckets = 128;
+
 __device__ inline uint32_t cost_bucket(float c)
 {
-    const float l = 2.f * __log2f(c + 1.f);
+    const float l = 8.f * __log2f(c + 1.f);
     const int k = static_cast<int>(l);
-    return 63u - static_cast<uint32_t>(k < 0 ? 0 : (k > 63 ? 63 : k));
+    return static_cast<uint32_t>(kSchedBuckets - 1) -
+           static_cast<uint32_t>(k < 0 ? 0 : (k > kSchedBuckets - 1 ? kSchedBuckets - 1 : k));
 }
 
-// Adds a block's bucket counts (LDS hist[64], complete) to the step's
-// counts; every thread of the block calls it.
+// Adds a block's bucket counts (LDS hist[kSchedBuckets], complete) to the
+// step's counts; every thread of the block calls it.
 __device__ inline void sched_hist_add(const TreeBuffers& tb, const uint32_t* hist)
 {
     __syncthreads();
-    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&tb.sc->sched_hist[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x < kSchedBuckets && hist[threadIdx.x])
+        atomicAdd(&tb.sc->sched_hist[threadIdx.x], hist[threadIdx.x]);
 }
 struct MixedParams {
     double theta2, G, soft2, half_dt;

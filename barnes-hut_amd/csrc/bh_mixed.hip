@@ -418,9 +418,10 @@ __device__ inline void interact(const float4 q, float ox, float oy, float oz, fl
 // Evaluates this lane's share of list[0..ln) for its body (offsets ox, oy,
 // oz): with K lanes per body, the entries sub, sub + K, sub + 2K, ...  FP32
 // FMA inside blocks of 32 such entries, Kahan-compensated across blocks.
+// Returns the wave's steps (entries per lane).
 template <int K>
-__device__ inline void flush_list_k(const float4* __restrict__ list, uint32_t ln, uint32_t sub, float ox, float oy,
-                                    float oz, float soft2, Acc3& a)
+__device__ inline uint32_t flush_list_k(const float4* __restrict__ list, uint32_t ln, uint32_t sub, float ox,
+                                        float oy, float oz, float soft2, Acc3& a)
 {
     ln = uni(ln);
     constexpr uint32_t span = 32u * K;
@@ -442,6 +443,7 @@ __device__ inline void flush_list_k(const float4* __restrict__ list, uint32_t ln
         for (uint32_t e = e0 + sub; e < ln; e += K) interact(list[e], ox, oy, oz, soft2, bx, by, bz);
         a.add(bx, by, bz);
     }
+    return (ln + K - 1) / K;
 }
 
 // p[0..R) (R wave-uniform) for one body: FP32 FMA inside blocks of 32
@@ -492,8 +494,8 @@ constexpr uint32_t kMinShare = 32;
 // 3 x 64 doubles per wave) and added in FP64 into (rx, ry, rz), in list
 // order.  Every body's sum depends on its group and list alone.  All passes
 // share one loop body, which keeps the kernel's code within the instruction
-// cache.
-__device__ inline void flush_list_1(const float4* __restrict__ list, uint32_t ln, uint32_t cnt, float ox, float oy,
+// cache.  Returns the wave's steps.
+__device__ inline uint32_t flush_list_1(const float4* __restrict__ list, uint32_t ln, uint32_t cnt, float ox, float oy,
                                     float oz, float soft2, Acc3& a, double (*hp)[64], double& rx, double& ry,
                                     double& rz)
 {
@@ -532,18 +534,19 @@ __device__ inline void flush_list_1(const float4* __restrict__ list, uint32_t ln
         }
         __builtin_amdgcn_wave_barrier();
     }
+    return ln - T;
 }
 
 // K (wave-uniform) = lanes per body: 4 for groups of <= 16 bodies, 2 for
 // <= 32, else 1 (then the lanes past the group's bodies help), so small
 // groups keep every lane busy.
-__device__ inline void flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub, uint32_t cnt,
-                                  float ox, float oy, float oz, float soft2, Acc3& a, double (*hp)[64], double& rx,
-                                  double& ry, double& rz)
+__device__ inline uint32_t flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub,
+                                      uint32_t cnt, float ox, float oy, float oz, float soft2, Acc3& a,
+                                      double (*hp)[64], double& rx, double& ry, double& rz)
 {
-    if (K == 4) flush_list_k<4>(list, ln, sub, ox, oy, oz, soft2, a);
-    else if (K == 2) flush_list_k<2>(list, ln, sub, ox, oy, oz, soft2, a);
-    else flush_list_1(list, ln, cnt, ox, oy, oz, soft2, a, hp, rx, ry, rz);
+    if (K == 4) return flush_list_k<4>(list, ln, sub, ox, oy, oz, soft2, a);
+    if (K == 2) return flush_list_k<2>(list, ln, sub, ox, oy, oz, soft2, a);
+    return flush_list_1(list, ln, cnt, ox, oy, oz, soft2, a, hp, rx, ry, rz);
 }
 
 constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
@@ -715,7 +718,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
         const float oz = static_cast<float>(z - cz);
         Acc3 acc;
         double rx = 0.0, ry = 0.0, rz = 0.0;   // the helpers' stretches (one lane per body)
-        uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0;
+        uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0, n_steps = 0;
         bool overflow = false;
         const uint32_t gsys = tb.bsys[b0];
         // heavy bodies of the system: FP64 direct sum in rank order
@@ -831,7 +834,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const uint32_t na = static_cast<uint32_t>(__popcll(am));
                 if (ln + na > static_cast<uint32_t>(kListCap)) {
                     __builtin_amdgcn_wave_barrier();
-                    flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
+                    n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
                     __builtin_amdgcn_wave_barrier();
                     ln = 0;
                 }
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         __builtin_amdgcn_wave_barrier();
-        flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
+        n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
         __builtin_amdgcn_wave_barrier();
         // the K partial sums of a body, added in FP64 in sub order (a Kahan
         // sum's value is s - c: c holds the negative of the rounding lost)
@@ -900,10 +903,14 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 b.vz[i] = b.vz[i] + p.half_dt * az;
             }
         }
-        if (valid && sub == 0) b.pcost[i] = n_inter;
+        // the group's work, the schedule's cost unit (a visit measured half
+        // the time of a list step on the 300k walk)
+        const uint32_t work = n_steps + n_visit / 2u;
+        if (valid && sub == 0) b.pcost[i] = work;
         if (lane == 0) {
             mb.g_inter[g] = n_inter;
             mb.g_visit[g] = n_visit;
+            mb.g_work[g] = work;
             mb.g_t0[g] = t_start;
             mb.g_t1[g] = wall_clock64();
             mb.g_worker[g] = worker;
@@ -964,23 +971,24 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
 // and the queue serves the costliest bucket first.  The order inside a bucket
 // is whatever the atomics give: it changes timing only, never results.
 //
-// Cost estimate, source 0: the previous step, i.e. the largest list length
-// any of the group's bodies saw (groups mostly persist from step to step).
+// Cost estimate, source 0: the previous step, i.e. the largest walk work of
+// the groups any of the group's bodies were in (groups mostly persist from
+// step to step).
 // Source 1: the cost model, evaluated by k_group_features.
 constexpr unsigned kSchedBlocks = 256;
 
 // Cost estimate from the previous step: one wavefront per group takes the
-// largest previous list length of its bodies and counts the group in its
+// largest previous walk work of its bodies and counts the group in its
 // block's bucket histogram, added to the step's counts.  The LEARNED form's
 // estimate comes from k_group_features instead, which ends the same way.
 __global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
                                                     uint32_t max_groups)
 {
-    __shared__ uint32_t hist[64];
+    __shared__ uint32_t hist[kSchedBuckets];
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
-    if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+    if (threadIdx.x < kSchedBuckets) hist[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t waves = gridDim.x * kWaves;
     for (uint32_t g = blockIdx.x * kWaves + (threadIdx.x >> 6); g < G; g += waves) {
@@ -995,22 +1003,28 @@ __global__ __launch_bounds__(kNT) void k_sched_cost(size_t n, Bodies b, TreeBuff
     sched_hist_add(tb, hist);
 }
 
-// Every block scans the step's 64 bucket counts into the buckets' first
-// places, then places tiles of 256 groups: LDS counters rank a tile's groups
-// inside their buckets, one global atomic per bucket reserves the tile's range.
+// Every block scans the step's bucket counts into the buckets' first places,
+// then places tiles of 256 groups: LDS counters rank a tile's groups inside
+// their buckets, one global atomic per bucket reserves the tile's range.
 __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb, MixedBuffers mb, uint32_t max_groups)
 {
-    __shared__ uint32_t cnt[64];
-    __shared__ uint32_t base[64];
-    __shared__ uint32_t first[64];
+    static_assert(kSchedBuckets == 128 && kNT >= 128, "two waves scan the bucket counts");
+    __shared__ uint32_t cnt[kSchedBuckets];
+    __shared__ uint32_t base[kSchedBuckets];
+    __shared__ uint32_t first[kSchedBuckets];
+    __shared__ uint32_t wave0_total;
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < kSchedBuckets) {
         const uint32_t v = tb.sc->sched_hist[threadIdx.x];
-        first[threadIdx.x] = wave_incl_scan(v) - v;
+        const uint32_t incl = wave_incl_scan(v);
+        if (threadIdx.x == 63) wave0_total = incl;
+        first[threadIdx.x] = incl - v;
     }
+    __syncthreads();
+    if (threadIdx.x >= 64 && threadIdx.x < kSchedBuckets) first[threadIdx.x] += wave0_total;
     for (uint32_t t0 = blockIdx.x * kNT; t0 < G; t0 += gridDim.x * kNT) {
-        if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+        if (threadIdx.x < kSchedBuckets) cnt[threadIdx.x] = 0;
         __syncthreads();
         const uint32_t g = t0 + threadIdx.x;
         uint32_t bk = 0, local = 0;
@@ -1019,7 +1033,7 @@ __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb,
             local = atomicAdd(&cnt[bk], 1u);
         }
         __syncthreads();
-        if (threadIdx.x < 64 && cnt[threadIdx.x])
+        if (threadIdx.x < kSchedBuckets && cnt[threadIdx.x])
             base[threadIdx.x] = first[threadIdx.x] + atomicAdd(&tb.sc->sched_cursor[threadIdx.x], cnt[threadIdx.x]);
         __syncthreads();
         if (g < G) mb.order[base[bk] + local] = g;
@@ -1077,7 +1091,7 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const MixedBuffers& mb, int source,
                    uint32_t n_chunks, hipStream_t s)
 {
-    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0xffffffffu);   // previous list lengths
+    if (source == 0) k_sched_cost<<<kSchedBlocks, kNT, 0, s>>>(n, b, tb, mb, 0xffffffffu);   // previous walk work
     // weights into mb.order, their exclusive prefix into mb.base (both free
     // once the node array is built)
     k_chunk_weights<<<nblk(n), kNT, 0, s>>>(n, tb, mb, mb.order);

@@ -1,11 +1,12 @@
 // bh_model.hip -- the learned cost model of the LEARNED form: a small MLP
-// that predicts the interaction-list length of every walk group, trained on
-// the device from the lengths the walk itself records.
+// that predicts the walk work of every walk group (list-evaluation steps +
+// visited nodes / 2), trained on the device from the work the walk itself
+// records.
 //
 // Network: 8 FP32 group features -> 32 -> 32 -> 1, ReLU on both hidden
 // layers, weights and biases stored in BF16 (round to nearest even from FP32
 // master copies), every product accumulated in FP32.  Output y is the scaled
-// log cost log2(list length + 1) / 16, formed as feature 5 (the same quantity
+// log cost log2(work + 1) / 16, formed as feature 5 (the same quantity
 // from the previous step) plus the network's output, so an untrained network
 // starts from the previous-step estimate and learns corrections to it.  Loss:
 // mean squared error on y.
@@ -68,7 +69,7 @@ __device__ inline float forward(const float* __restrict__ P, const float* x, Act
         for (int k = 0; k < kH; ++k) s = fmaf(P[kOffW2 + j * kH + k], relu(a.z1[k]), s);
         a.z2[j] = s;
     }
-    // residual on feature 5 (the previous step's scaled list length): the
+    // residual on feature 5 (the previous step's scaled walk work): the
     // network learns the change from the fallback estimate
     float y = P[kOffB3];
     for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(a.z2[j]), y);
@@ -118,7 +119,7 @@ __device__ inline float forward_wave(const float* __restrict__ P, const float* x
 
 // One wavefront per group (grid-stride): lanes hold the group's bodies.  With
 // the model's weights (wb), the wavefront also evaluates the model on the
-// group's features (gcost = 2^(16 y) - 1, the predicted list length), counts
+// group's features (gcost = 2^(16 y) - 1, the predicted walk work), counts
 // the group in its block's bucket histogram, and the last block scans the
 // buckets for the heaviest-first order (sched_hist_add).
 __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
@@ -126,12 +127,12 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
                                                         const uint16_t* __restrict__ wb)
 {
     __shared__ float P[kModelParams];
-    __shared__ uint32_t hist[64];
+    __shared__ uint32_t hist[kSchedBuckets];
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
     if (wb) {
-        if (threadIdx.x < 64) hist[threadIdx.x] = 0;
+        if (threadIdx.x < kSchedBuckets) hist[threadIdx.x] = 0;
         load_params(wb, P);   // ends with a barrier
     }
     const uint32_t waves = gridDim.x * (kNT / 64);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kNT) void k_model_forward(const float* __restrict__
 
 // ------------------------------------------------------------- samples
 // Appends the groups of this step to the ring buffer: features and target
-// log2(list length + 1) / 16.
+// log2(work + 1) / 16, work = the walk's list steps + visits / 2.
 __global__ __launch_bounds__(kNT) void k_append_samples(const float* __restrict__ feat, TreeBuffers tb,
                                                         MixedBuffers mb, ModelBuffers md)
 {
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kNT) void k_append_samples(const float* __restrict_
     const uint64_t slot = (base + g) % md.capacity;
 #pragma unroll
     for (int f = 0; f < kF; ++f) md.sfeat[kF * slot + f] = feat[kF * g + f];
-    md.starget[slot] = flog2(static_cast<float>(mb.g_inter[g]) + 1.f) / 16.f;
+    md.starget[slot] = flog2(static_cast<float>(mb.g_work[g]) + 1.f) / 16.f;
 }
 
 __global__ void k_samples_commit(TreeBuffers tb, ModelBuffers md) { *md.n_seen += tb.sc->n_groups; }

@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
         sc->heavy_counter = 0;
     }
     if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < kSchedBuckets) {
         sc->sched_hist[threadIdx.x] = 0;
         sc->sched_cursor[threadIdx.x] = 0;
     }

@@ -153,7 +153,8 @@ int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
 /* Order in which the persistent mixed walk serves its groups (results never
  * depend on it):
  *   KEY_ORDER  dynamic queue in Morton order (default of MIXED)
- *   PREV_COST  heaviest first by the list lengths measured in the last step
+ *   PREV_COST  heaviest first by the walk work measured in the last step
+ *              (list-evaluation steps + visited nodes / 2 per group)
  *   MODEL      heaviest first by the learned cost model (LEARNED; falls back
  *              to PREV_COST while no model is trained) */
 #define BH_SCHED_KEY_ORDER 0
@@ -185,17 +186,18 @@ int bh_get_sort_stats(bh_ctx *ctx, int *near_path, int64_t *sorted_keys);
 
 /* ------------------------------------------------ learned cost model ---
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
- * accumulation) predicts log2(list length + 1) / 16 of every walk group as
+ * accumulation) predicts log2(work + 1) / 16 of every walk group (work =
+ * list-evaluation steps + visited nodes / 2, as the walk records it) as
  * feature 5 (the previous step's value) plus its output; the
  * LEARNED form serves groups heaviest-first by its predictions.  It trains on
- * the device from the list lengths the walk records, during the first
+ * the device from the work the walk records, during the first
  * train_steps LEARNED evaluations (iters_per_step Adam updates of `batch`
  * samples each), and is used from then on.  Features per group:
  *   [0] tree depth / 21         [1] bodies / 64
  *   [2] log2(box extent) / 16   [3] log2(distance of box centre to origin) / 8
  *   [4] (log10(light mass + 1) - 20) / 5
- *   [5] log2(max list length of its bodies last step + 1) / 16
- *   [6] log2(min list length of its bodies last step + 1) / 16
+ *   [5] log2(max walk work of its bodies' groups last step + 1) / 16
+ *   [6] log2(min walk work of its bodies' groups last step + 1) / 16
  *   [7] log2(z extent / box extent) / 8 */
 #define BH_MODEL_FEATURES 8
 #define BH_MODEL_PARAMS 1377
@@ -238,6 +240,12 @@ int bh_get_groups(bh_ctx *ctx, int64_t *n_groups, int64_t capacity, uint32_t *fi
  * (100 MHz ticks) when a worker wave took it and when it finished, and the
  * worker's index.  Arrays sized by the group count of bh_get_groups. */
 int bh_get_group_timeline(bh_ctx *ctx, int64_t capacity, uint64_t *t_start, uint64_t *t_end, uint32_t *worker);
+
+/* Walk work of each group of the last mixed walk: its list-evaluation steps
+ * (list entries per lane) + visited nodes / 2, the cost unit of the
+ * heaviest-first schedules and the learned model's target.  Array sized by
+ * the group count of bh_get_groups. */
+int bh_get_group_work(bh_ctx *ctx, int64_t capacity, uint32_t *work);
 
 /* FP64 direct sum on the device for m sample bodies (indices into the
  * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 over the

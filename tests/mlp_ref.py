@@ -35,7 +35,7 @@ def forward(p, x, dtype=np.float32, round_bf16=True):
     h1 = np.maximum(z1, 0)
     z2 = h1 @ q["W2"].astype(dtype).T + q["b2"].astype(dtype)
     h2 = np.maximum(z2, 0)
-    # residual on feature 5 (previous step's scaled list length)
+    # residual on feature 5 (previous step's scaled walk work)
     y = h2 @ q["W3"].astype(dtype) + q["b3"].astype(dtype)[0] + x[:, 5]
     return y, (x, z1, h1, z2, h2)
 

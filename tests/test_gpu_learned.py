@@ -86,7 +86,7 @@ def test_learned_forces_bit_identical_to_mixed(bh):
 def test_learned_gain_over_schedules_without_it(bh):
     """Trains the model on 50 steps of a Trojan-rich system (strongly uneven
     walk cost), then measures, on the following steps, the prediction error of
-    the network against the previous-step counts and the walk time of the
+    the network against the previous-step work and the walk time of the
     three schedules.  The figures go to gpurun_out/learned_gain.json."""
     n = 262144
     pos, vel, m = _trojan(bh, n, seed=5)
@@ -97,18 +97,18 @@ def test_learned_gain_over_schedules_without_it(bh):
         ready, samples, updates, loss = eng.model_status()
         assert ready
         say("trained: samples", samples, "updates", updates, "loss", loss)
-        # prediction quality on the next step: network vs previous counts
+        # prediction quality on the next step: network vs previous work
         eng.step(1, HOUR)
         feat = eng.group_features()
-        first, inter, visit = eng.groups()
+        work = eng.group_work()
         pred = 2.0 ** (16 * eng.model_forward(feat)) - 1
         prev = 2.0 ** (16 * feat[:, 5]) - 1
-        tgt = inter.astype(np.float64)
+        tgt = work.astype(np.float64)
         e_net = float(np.mean(np.abs(np.log2(pred + 1) - np.log2(tgt + 1))))
         e_prev = float(np.mean(np.abs(np.log2(prev + 1) - np.log2(tgt + 1))))
         e_const = float(np.mean(np.abs(np.log2(tgt + 1) - np.mean(np.log2(tgt + 1)))))
         out.update(mean_abs_log2_err_network=e_net, mean_abs_log2_err_prev_counts=e_prev,
-                   mean_abs_log2_err_constant=e_const, groups=int(len(inter)), train_loss=float(loss))
+                   mean_abs_log2_err_constant=e_const, groups=int(len(work)), train_loss=float(loss))
         say("log2 error: network", e_net, "prev", e_prev, "constant", e_const)
         assert e_net < e_const
         state = eng.get_bodies()
