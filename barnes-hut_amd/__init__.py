@@ -108,6 +108,7 @@ SIGNATURES = [
     ("bh_set_relative_opening", C.c_int, [_P, _D]),
     ("bh_set_schedule_split", C.c_int, [_P, C.c_int]),
     ("bh_set_sort_mode", C.c_int, [_P, C.c_int]),
+    ("bh_set_moment_cutoff", C.c_int, [_P, C.c_int]),
     ("bh_get_sort_stats", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_I64)]),
     ("bh_get_relative_opening", C.c_int, [_P, C.POINTER(_D)]),
 ]
@@ -318,6 +319,11 @@ class Engine:
 
     def set_schedule_split(self, chunks_per_worker: int):
         self._ck(self.lib.bh_set_schedule_split(self.h, chunks_per_worker))
+
+    def set_moment_cutoff(self, cutoff: int):
+        """Depth from which one workgroup sums the tree moments in a single
+        launch (-1: chosen when a step graph is captured; 21: none)."""
+        self._ck(self.lib.bh_set_moment_cutoff(self.h, cutoff))
 
     def set_sort_mode(self, mode: int):
         """SORT_NEAR (default): sort only the keys out of order since the

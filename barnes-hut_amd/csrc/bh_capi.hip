@@ -60,6 +60,7 @@ struct bh_ctx {
     int sched = BH_SCHED_KEY_ORDER;   // schedule of the MIXED form
     int split = 0;                    // chunks per worker of the split schedule (0: heaviest first)
     bool sort_near = true;            // key sort: near-sorted front end (false: full radix sort)
+    int moment_cut = -1;              // deep cutoff of the tree moments (-1: chosen at graph capture)
     bool model_ready = false;         // the LEARNED cost model is trained
     // cost model (allocated once per context, independent of N)
     ModelBuffers md{};
@@ -171,7 +172,7 @@ int reserve(bh_ctx* c, size_t n)
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
     A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.lsec, 2 * N); A(t.walk, 2 * N);
-    A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems);
+    A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems); A(t.deep, N);
     t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gfirst, N + 1); A(m.gdepth, N);
@@ -477,6 +478,36 @@ std::vector<double> graph_key(const bh_ctx* c, double dt)
             static_cast<double>(c->mixed_blocks), static_cast<double>(c->cap)};
 }
 
+// Deep cutoff of the tree moments: one below the deepest branching node of
+// the last tree (at least 1).  The depths from there down hold no node when
+// the graph is captured: one workgroup launch (k_moments_deep) stands in for
+// their launches and sums whatever nodes they gain later, level by level.
+// Every cutoff gives the same moments.  (Measured on the 300k learned step:
+// cutoff 21, one launch per depth, 0.844 ms; 14, the first empty depth,
+// 0.833 ms; 12, two sparse depths in the workgroup, 0.836 ms; 10, 0.939 ms:
+// a depth with nodes costs the workgroup a memory round trip, more than its
+// own launch, which costs about 1.5 us.)
+
+int choose_deep_from(bh_ctx* c)
+{
+    if (c->moment_cut >= 0) {
+        c->tb.deep_from = c->moment_cut;
+        return BH_OK;
+    }
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> depth(sc.n_branch);
+    if (sc.n_branch)
+        HIP_TRY(c, hipMemcpy(depth.data(), c->tb.bdepth, sc.n_branch, hipMemcpyDeviceToHost));
+    uint64_t hist[bh::kMaxDepth + 1] = {};
+    for (const uint8_t d : depth) ++hist[d < bh::kMaxDepth ? d : bh::kMaxDepth];
+    int from = bh::kMaxDepth;
+    while (from > 1 && hist[from - 1] == 0) --from;
+    c->tb.deep_from = from;
+    return BH_OK;
+}
+
 // One steady step through the graph of the current parity (captured on first use).
 int graph_step(bh_ctx* c, double dt)
 {
@@ -485,6 +516,7 @@ int graph_step(bh_ctx* c, double dt)
     if (!c->gexec[p] || c->gkey[p] != key) {
         if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
         c->gexec[p] = nullptr;
+        if (const int rc0 = choose_deep_from(c)) return rc0;
         hipGraph_t g = nullptr;
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         const int rc = run_force(c, c->cfg.form, c->cfg.theta, c->cfg.G, c->cfg.soft2, true, true, dt, true, nullptr);
@@ -1090,6 +1122,17 @@ int bh_set_sort_mode(bh_ctx* c, int mode)
     if (check_ctx(c)) return BH_ERR_ARG;
     if (mode != BH_SORT_NEAR && mode != BH_SORT_FULL) return fail(c, BH_ERR_ARG, "unknown sort mode %d", mode);
     c->sort_near = mode == BH_SORT_NEAR;
+    return BH_OK;
+}
+
+int bh_set_moment_cutoff(bh_ctx* c, int cutoff)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    if (cutoff < -1 || cutoff > bh::kMaxDepth) return fail(c, BH_ERR_ARG, "moment cutoff %d outside [-1, %d]", cutoff,
+                                                          bh::kMaxDepth);
+    c->moment_cut = cutoff;
+    if (cutoff >= 0) c->tb.deep_from = cutoff;
+    drop_graphs(c);
     return BH_OK;
 }
 

@@ -82,6 +82,7 @@ struct StepScalars {
     uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
     uint32_t sched_hist[128];   // groups per cost bucket of the heaviest-first order
     uint32_t sched_cursor[128]; // groups placed per bucket so far (both cleared by k_kick_drift_box)
+    uint32_t n_deep;            // branching nodes at depth >= TreeBuffers::deep_from (listed in deep)
 };
 
 // In key order the mixed walk deals its groups out in runs of kWalkRun
@@ -218,6 +219,11 @@ struct TreeBuffers {
     const uint8_t* bsys;  // system of each body position (fixed; all 0 for one system)
     uint32_t n_sys;
     uint32_t box_blocks;  // k_kick_drift_box's grid: blocks per system in proportion to its bodies
+    // Branching nodes at depth >= deep_from get their moments from one
+    // single-block launch (k_moments_deep) instead of one launch per depth;
+    // k_branch_init lists them in deep (capacity N).  kMaxDepth: none.
+    uint32_t* deep;
+    int deep_from = kMaxDepth;
 };
 
 // kick (optional) + drift + per-system box; writes bb/span/side2 of every

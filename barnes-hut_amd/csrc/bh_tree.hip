@@ -225,6 +225,7 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
         sc->group_counter = 0;
         sc->block_counter = 0;
         sc->heavy_counter = 0;
+        sc->n_deep = 0;
     }
     if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
     if (threadIdx.x < kSchedBuckets) {
@@ -426,6 +427,18 @@ __global__ __launch_bounds__(kNT) void k_branch_init(size_t n, TreeBuffers tb)
     int32_t* c = tb.child + 8 * static_cast<size_t>(bi);
 #pragma unroll
     for (int k = 0; k < 8; ++k) c[k] = -1;
+    // the deep list, one atomic per wavefront (over the lanes still here)
+    const bool deep = d >= tb.deep_from;
+    const uint64_t dm = __ballot(deep);
+    if (deep) {
+        const int lane = static_cast<int>(threadIdx.x & 63);
+        const int lead = __ffsll(static_cast<unsigned long long>(dm)) - 1;
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(&tb.sc->n_deep, static_cast<uint32_t>(__popcll(dm)));
+        base = static_cast<uint32_t>(__shfl(static_cast<int>(base), lead, 64));
+        const uint32_t below = static_cast<uint32_t>(__popcll(dm & ((1ull << lane) - 1ull)));
+        tb.deep[base + below] = bi;
+    }
 }
 
 // parent links and octant slots of every node
@@ -522,6 +535,38 @@ __global__ __launch_bounds__(kNT) void k_moments_level(TreeBuffers tb, int depth
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
     if (bi >= B || tb.bdepth[bi] != depth) return;
     node_moments(tb, M, bi);
+}
+
+// The listed deep nodes (depth >= deep_from), bottom-up by depth in one
+// workgroup: a barrier between depths makes a level's moments visible to the
+// next, so each node sees final children, as with one launch per depth.  The
+// list is in LDS when it fits, else read from memory on every level.
+constexpr int kDeepNT = 1024;
+constexpr uint32_t kDeepLds = 4096;
+
+__global__ __launch_bounds__(kDeepNT) void k_moments_deep(TreeBuffers tb)
+{
+    __shared__ uint32_t sn[kDeepLds];
+    __shared__ uint8_t sd[kDeepLds];
+    const uint32_t M = tb.sc->n_leaves, nd = tb.sc->n_deep;
+    if (nd == 0) return;
+    const bool lds = nd <= kDeepLds;
+    if (lds) {
+        for (uint32_t k = threadIdx.x; k < nd; k += kDeepNT) {
+            const uint32_t bi = tb.deep[k];
+            sn[k] = bi;
+            sd[k] = tb.bdepth[bi];
+        }
+        __syncthreads();
+    }
+    for (int d = kMaxDepth - 1; d >= tb.deep_from; --d) {
+        for (uint32_t k = threadIdx.x; k < nd; k += kDeepNT) {
+            const uint32_t bi = lds ? sn[k] : tb.deep[k];
+            const int dk = lds ? sd[k] : tb.bdepth[bi];
+            if (dk == d) node_moments(tb, M, bi);
+        }
+        __syncthreads();
+    }
 }
 
 // reversed per-leaf node counts for the suffix scan
@@ -627,7 +672,9 @@ void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb,
 
 void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s)
 {
-    for (int d = kMaxDepth - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
+    const int from = tb.deep_from < kMaxDepth ? tb.deep_from : kMaxDepth;
+    if (from < kMaxDepth) k_moments_deep<<<1, kDeepNT, 0, s>>>(tb);
+    for (int d = from - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
 }
 
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s)

@@ -173,16 +173,27 @@ int bh_set_schedule_split(bh_ctx *ctx, int chunks_per_worker);
 /* Key sort of the step (no counterpart in the reference, whose
  * sortBodiesByMortonKey, include/utility.h:23, is a full stable sort every
  * step).  BH_SORT_NEAR (default): the bodies stay in the previous step's
- * order, so only the keys out of order with their neighbours are radix
- * sorted and merged with the rest; when the rest is not in order (or more
- * than half the keys moved) every key is sorted.  BH_SORT_FULL: always the
- * full radix sort.  Both give the same, stable order. */
+ * order, so only the keys out of order with their neighbours are sorted (a
+ * merge sort) and merged with the rest; when the rest is not in order (or
+ * more than half the keys moved) the merge sort takes every key.
+ * BH_SORT_FULL: always the full radix sort.  Both give the same, stable
+ * order. */
 #define BH_SORT_NEAR 0
 #define BH_SORT_FULL 1
 int bh_set_sort_mode(bh_ctx *ctx, int mode);
 /* Path of the last key sort: *near_path = 1 if it took the near-sorted path,
  * *sorted_keys = the keys it radix sorted (n otherwise).  Synchronises. */
 int bh_get_sort_stats(bh_ctx *ctx, int *near_path, int64_t *sorted_keys);
+
+/* Tree moments (no counterpart in the reference, whose octree sums them in
+ * one recursive pass, src/linear_octree.cpp): one launch per depth, bottom
+ * up, except that the branching nodes at depth >= cutoff are summed by a
+ * single workgroup, depth by depth, in one launch.  cutoff -1 (default):
+ * chosen from the current tree whenever a step graph is captured, one below
+ * its deepest branching node (so the empty depths cost one launch, not one
+ * each); 0..21 fixes it (21: one launch per depth throughout).  Every cutoff
+ * gives the same bits. */
+int bh_set_moment_cutoff(bh_ctx *ctx, int cutoff);
 
 /* ------------------------------------------------ learned cost model ---
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
