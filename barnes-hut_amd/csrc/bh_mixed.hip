@@ -490,14 +490,13 @@ constexpr uint32_t kMinShare = 32;
 // body lanes take the entries [0, T) of their body in q stretches of R, while
 // idle lane e takes [T, T + R) of bodies e, e + E, e + 2E, ... in the same
 // stretches; the body lanes then take [T + R, ln).  A helper's stretch is a
-// compensated sum of its own, handed to the body's lane through hp (LDS,
-// 3 x 64 doubles per wave) and added in FP64 into (rx, ry, rz), in list
-// order.  Every body's sum depends on its group and list alone.  All passes
-// share one loop body, which keeps the kernel's code within the instruction
-// cache.  Returns the wave's steps.
+// compensated sum of its own, handed to the body's lane by a cross-lane read
+// (body b of stretch s reads lane cnt + b - sE) and added in FP64 into
+// (rx, ry, rz), in list order.  Every body's sum depends on its group and
+// list alone.  All passes share one loop body, which keeps the kernel's code
+// within the instruction cache.  Returns the wave's steps.
 __device__ inline uint32_t flush_list_1(const float4* __restrict__ list, uint32_t ln, uint32_t cnt, float ox, float oy,
-                                    float oz, float soft2, Acc3& a, double (*hp)[64], double& rx, double& ry,
-                                    double& rz)
+                                        float oz, float soft2, Acc3& a, double& rx, double& ry, double& rz)
 {
     ln = uni(ln);
     const uint32_t lane = __lane_id();
@@ -519,20 +518,19 @@ __device__ inline uint32_t flush_list_1(const float4* __restrict__ list, uint32_
         if (helper && !last) a = Acc3();
         const uint32_t from = last ? T + (q ? R : 0u) : (helper ? T : s * R);
         seg_sum(list + from, last ? ln - T - (q ? R : 0u) : R, px, py, pz, soft2, a);
-        if (hv) {
-            hp[0][hb] = static_cast<double>(a.sx) - static_cast<double>(a.cx);
-            hp[1][hb] = static_cast<double>(a.sy) - static_cast<double>(a.cy);
-            hp[2][hb] = static_cast<double>(a.sz) - static_cast<double>(a.cz);
+        if (!last) {
+            // bodies sE .. sE + E - 1 take their stretch from lanes cnt .. cnt + E - 1
+            const bool own = !helper && lane >= s * E && lane < s * E + E;
+            const int from_lane = static_cast<int>(own ? cnt + lane - s * E : lane);
+            const double gx = __shfl(static_cast<double>(a.sx) - static_cast<double>(a.cx), from_lane, 64);
+            const double gy = __shfl(static_cast<double>(a.sy) - static_cast<double>(a.cy), from_lane, 64);
+            const double gz = __shfl(static_cast<double>(a.sz) - static_cast<double>(a.cz), from_lane, 64);
+            if (own) {
+                rx += gx;
+                ry += gy;
+                rz += gz;
+            }
         }
-    }
-    if (q) {
-        __builtin_amdgcn_wave_barrier();
-        if (!helper) {
-            rx += hp[0][lane];
-            ry += hp[1][lane];
-            rz += hp[2][lane];
-        }
-        __builtin_amdgcn_wave_barrier();
     }
     return ln - T;
 }
@@ -541,12 +539,12 @@ __device__ inline uint32_t flush_list_1(const float4* __restrict__ list, uint32_
 // <= 32, else 1 (then the lanes past the group's bodies help), so small
 // groups keep every lane busy.
 __device__ inline uint32_t flush_list(const float4* __restrict__ list, uint32_t ln, uint32_t K, uint32_t sub,
-                                      uint32_t cnt, float ox, float oy, float oz, float soft2, Acc3& a,
-                                      double (*hp)[64], double& rx, double& ry, double& rz)
+                                      uint32_t cnt, float ox, float oy, float oz, float soft2, Acc3& a, double& rx,
+                                      double& ry, double& rz)
 {
     if (K == 4) return flush_list_k<4>(list, ln, sub, ox, oy, oz, soft2, a);
     if (K == 2) return flush_list_k<2>(list, ln, sub, ox, oy, oz, soft2, a);
-    return flush_list_1(list, ln, cnt, ox, oy, oz, soft2, a, hp, rx, ry, rz);
+    return flush_list_1(list, ln, cnt, ox, oy, oz, soft2, a, rx, ry, rz);
 }
 
 constexpr int kNodeBuf = 512;    // nodes of one round (<= 64 ranges of <= 8)
@@ -634,10 +632,8 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
     __shared__ float4 s_list[kWaves][kListCap];
     __shared__ uint32_t s_stack[kWaves][kStackCap];
     __shared__ uint32_t s_nodes[kWaves][kNodeBuf];
-    __shared__ double s_help[kWaves][3][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float4* list = s_list[w];
-    double(*hp)[64] = s_help[w];
     uint32_t* stk = s_stack[w];
     uint32_t* nbuf = s_nodes[w];
     const uint32_t G = tb.sc->n_groups;
@@ -834,7 +830,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const uint32_t na = static_cast<uint32_t>(__popcll(am));
                 if (ln + na > static_cast<uint32_t>(kListCap)) {
                     __builtin_amdgcn_wave_barrier();
-                    n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
+                    n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, rx, ry, rz);
                     __builtin_amdgcn_wave_barrier();
                     ln = 0;
                 }
@@ -870,7 +866,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         __builtin_amdgcn_wave_barrier();
-        n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, hp, rx, ry, rz);
+        n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, rx, ry, rz);
         __builtin_amdgcn_wave_barrier();
         // the K partial sums of a body, added in FP64 in sub order (a Kahan
         // sum's value is s - c: c holds the negative of the rounding lost)
