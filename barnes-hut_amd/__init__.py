@@ -85,6 +85,7 @@ SIGNATURES = [
     ("bh_set_schedule", C.c_int, [_P, C.c_int]),
     ("bh_get_group_timeline", C.c_int, [_P, _I64, _P, _P, _P]),
     ("bh_get_group_work", C.c_int, [_P, _I64, _P]),
+    ("bh_get_group_costs", C.c_int, [_P, _I64, _P]),
     ("bh_model_configure", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_float]),
     ("bh_model_reset", C.c_int, [_P, C.c_uint64]),
     ("bh_model_get_params", C.c_int, [_P, _P]),
@@ -449,6 +450,14 @@ class Engine:
         work = np.zeros(k.value, np.uint32)
         self._ck(self.lib.bh_get_group_work(self.h, k.value, _ptr(work)))
         return work
+
+    def group_costs(self):
+        """Estimated cost per group that the last walk's heaviest-first order used."""
+        k = _I64(0)
+        self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
+        cost = np.zeros(k.value, np.float32)
+        self._ck(self.lib.bh_get_group_costs(self.h, k.value, _ptr(cost)))
+        return cost
 
     def groups(self):
         """Walk groups of the last mixed evaluation: first_body (G+1),

@@ -1193,6 +1193,19 @@ int bh_get_group_work(bh_ctx* c, int64_t capacity, uint32_t* work)
     return BH_OK;
 }
 
+int bh_get_group_costs(bh_ctx* c, int64_t capacity, float* cost)
+{
+    if (check_ctx(c) || !cost) return BH_ERR_ARG;
+    if (!c->have_groups) return fail(c, BH_ERR_STATE, "no mixed evaluation yet");
+    StepScalars sc;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(&sc, c->tb.sc, sizeof sc, hipMemcpyDeviceToHost));
+    const size_t G = sc.n_groups;
+    if (capacity < static_cast<int64_t>(G)) return fail(c, BH_ERR_OVERFLOW, "%zu groups", G);
+    HIP_TRY(c, hipMemcpy(cost, c->mb.gcost, G * 4, hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
 int bh_direct_sum(bh_ctx* c, int64_t m, const int64_t* idx, double G, double soft2, bh_vec3* acc_out)
 {
     if (check_ctx(c)) return BH_ERR_ARG;

@@ -86,29 +86,43 @@ __device__ inline T wave_reduce(T v, Op op)
     return v;
 }
 
+// The parameters for forward_wave: as load_params, but W1 and W2 stored
+// transposed ([input][unit]), so the lanes of a layer read consecutive words
+// (in the stored [unit][input] order 32 lanes read one LDS bank).
+__device__ inline void load_params_by_unit(const uint16_t* __restrict__ wb, float* s)
+{
+    for (int k = threadIdx.x; k < kModelParams; k += blockDim.x) {
+        int d = k;
+        if (k < kOffB1) d = kOffW1 + (k - kOffW1) % kF * kH + (k - kOffW1) / kF;
+        else if (k >= kOffW2 && k < kOffB2) d = kOffW2 + (k - kOffW2) % kH * kH + (k - kOffW2) / kH;
+        s[d] = from_bf16(wb[k]);
+    }
+    __syncthreads();
+}
+
 __device__ inline float lane_f32(float v, int k)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
 // forward() spread over a wavefront for one group (x: the same features in
-// every lane): lane j < 32 computes hidden unit j of each layer, the output
-// runs over the units in order -- every sum in forward()'s order, so the
-// same bits.
+// every lane; P from load_params_by_unit): lane j < 32 computes hidden unit j
+// of each layer, the output runs over the units in order -- every sum in
+// forward()'s order, so the same bits.
 __device__ inline float forward_wave(const float* __restrict__ P, const float* x, int lane)
 {
     float z1 = 0.f, z2 = 0.f;
     if (lane < kH) {
         float s = P[kOffB1 + lane];
 #pragma unroll
-        for (int f = 0; f < kF; ++f) s = fmaf(P[kOffW1 + lane * kF + f], x[f], s);
+        for (int f = 0; f < kF; ++f) s = fmaf(P[kOffW1 + f * kH + lane], x[f], s);
         z1 = s;
     }
     {
         float s = lane < kH ? P[kOffB2 + lane] : 0.f;
         for (int k = 0; k < kH; ++k) {
             const float a = relu(lane_f32(z1, k));
-            if (lane < kH) s = fmaf(P[kOffW2 + lane * kH + k], a, s);
+            if (lane < kH) s = fmaf(P[kOffW2 + k * kH + lane], a, s);
         }
         z2 = s;
     }
@@ -133,7 +147,7 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
     const int lane = threadIdx.x & 63;
     if (wb) {
         if (threadIdx.x < kSchedBuckets) hist[threadIdx.x] = 0;
-        load_params(wb, P);   // ends with a barrier
+        load_params_by_unit(wb, P);   // ends with a barrier
     }
     const uint32_t waves = gridDim.x * (kNT / 64);
     auto dmin = [](double a, double c) { return c < a ? c : a; };

@@ -258,6 +258,12 @@ int bh_get_group_timeline(bh_ctx *ctx, int64_t capacity, uint64_t *t_start, uint
  * the group count of bh_get_groups. */
 int bh_get_group_work(bh_ctx *ctx, int64_t capacity, uint32_t *work);
 
+/* Estimated cost of each group of the last mixed walk, as its heaviest-first
+ * order used it: the learned model's prediction 2^(16 y) - 1 (LEARNED) or the
+ * previous step's work (PREV_COST).  Meaningful only for a walk that served
+ * such an order.  Array sized by the group count of bh_get_groups. */
+int bh_get_group_costs(bh_ctx *ctx, int64_t capacity, float *cost);
+
 /* FP64 direct sum on the device for m sample bodies (indices into the
  * current body order): a_i = sum_{j != i, r > 0} G m_j d / |d|^3 over the
  * bodies j of i's system, with Neumaier-compensated sums, r^2 = |d|^2 +

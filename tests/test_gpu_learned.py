@@ -55,6 +55,27 @@ def test_model_training_reduces_loss(bh):
         assert l1 < 0.1 * l0
 
 
+def test_schedule_costs_are_the_model_predictions(bh):
+    """The features kernel evaluates the model once per group, spread over a
+    wavefront with the weights in LDS by unit; its cost estimates must be the
+    model's predictions (forward(), same operation order) on the same
+    features."""
+    pos, vel, m = _trojan(bh, 60000, seed=4)
+    with bh.Engine(form=bh.FORM_LEARNED, theta=1.05) as eng:
+        eng.set_graph(False)
+        eng.model_configure(train_steps=4, iters_per_step=4, batch=512, lr=3e-3)
+        eng.set_bodies(pos, vel, m)
+        eng.step(6, HOUR)
+        assert eng.model_status()[0]
+        feat = eng.group_features()
+        cost = eng.group_costs()
+        y = eng.model_forward(feat)
+    assert np.abs(y - feat[:, 5]).max() > 0   # the network's correction is not zero
+    want = np.maximum(np.exp2(np.float32(16) * np.minimum(y, np.float32(1.5))) - np.float32(1), np.float32(0))
+    say("groups", len(cost), "max rel diff", float(np.max(np.abs(cost - want) / np.maximum(want, 1))))
+    assert np.allclose(cost, want, rtol=4e-6, atol=1e-3)
+
+
 def _trojan(bh, n, seed=3):
     return bh.generate(bh.GEN_TROJAN, seed, n)
 
