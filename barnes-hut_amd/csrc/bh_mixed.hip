@@ -600,13 +600,15 @@ __device__ inline void heavy_item(const Bodies& b, const TreeBuffers& tb, const 
     const size_t j0 = sy->body_lo + static_cast<size_t>(c) * kHeavyChunk;
     const size_t j1 = min(j0 + kHeavyChunk, static_cast<size_t>(sy->body_hi));
     Neumaier ax, ay, az;
+    // the heavy body itself and co-located bodies add an exact zero (a no-op
+    // for the sums) instead of being skipped, so the loop unrolls and the
+    // square roots and divisions of consecutive bodies overlap
+#pragma unroll 4
     for (size_t j = j0 + lane; j < j1; j += 64) {
-        if (j == h) continue;
         const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
         const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
-        if (!(r2 > 0.0)) continue;
         const double invR = 1.0 / sqrt(r2);
-        const double s = p.G * b.m[j] * (invR * invR * invR);
+        const double s = (j != h && r2 > 0.0) ? p.G * b.m[j] * (invR * invR * invR) : 0.0;
         ax.add(s * dx);
         ay.add(s * dy);
         az.add(s * dz);
