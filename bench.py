@@ -318,8 +318,8 @@ def main():
     else:
         kname, kern, peak, bound_note = "k_walk_mixed", \
             "k_walk_mixed (FP32 group walk + FP64 heavy term + fused half kick)", FP32_PEAK_TFLOPS, \
-            "FP32 vector (published peak counts packed FP32, which this build does not use: the non-packed " \
-            "ceiling is 78.6)"
+            "FP32 vector (the published peak counts packed FP32 instructions, which the library's kernels are " \
+            "compiled without: the non-packed ceiling is 78.6)"
     tr = traffic_record(kname, args.config)
     roofline = {"bound": "mfma", "achieved": head["walk_tflops"], "peak": peak, "unit": "TFLOP/s",
                 "frac": head["walk_tflops"] / peak, "traffic": tr["bytes_per_launch"] if tr else None,

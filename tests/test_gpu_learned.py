@@ -3,6 +3,7 @@ the numpy reference (tests/mlp_ref.py, itself pinned by finite differences),
 training on the device, forces bit-identical to the MIXED schedule, and what
 the trained network gains over the schedules without it."""
 import json
+import os
 import pathlib
 import sys
 import time
@@ -14,7 +15,8 @@ import mlp_ref
 
 pytestmark = pytest.mark.gpu
 HOUR = 1.0 / 24.0
-OUT = pathlib.Path(__file__).resolve().parents[1] / "gpurun_out"
+# directory for the measured figures of the gain test (nothing is written when unset)
+OUT = os.environ.get("BH_TEST_OUT")
 
 
 def say(*a):
@@ -108,7 +110,8 @@ def test_learned_gain_over_schedules_without_it(bh):
     """Trains the model on 50 steps of a Trojan-rich system (strongly uneven
     walk cost), then measures, on the following steps, the prediction error of
     the network against the previous-step work and the walk time of the
-    three schedules.  The figures go to gpurun_out/learned_gain.json."""
+    three schedules.  The figures go to $BH_TEST_OUT/learned_gain.json when
+    that variable names a directory."""
     n = 262144
     pos, vel, m = _trojan(bh, n, seed=5)
     out = {}
@@ -152,6 +155,7 @@ def test_learned_gain_over_schedules_without_it(bh):
         ph, k = e3.step_phase_ms()
         out["walk_ms_network"] = float(ph[3])
     say("walk ms", {k: v for k, v in out.items() if k.startswith("walk")})
-    OUT.mkdir(exist_ok=True)
-    (OUT / "learned_gain.json").write_text(json.dumps(out, indent=1))
+    if OUT:
+        pathlib.Path(OUT).mkdir(parents=True, exist_ok=True)
+        (pathlib.Path(OUT) / "learned_gain.json").write_text(json.dumps(out, indent=1))
     assert out["walk_ms_network"] < 1.10 * out["walk_ms_key_order"]
