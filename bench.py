@@ -226,6 +226,9 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only time the CPU baseline of the config (--cpu-steps steps; an ensemble: that many "
+                         "steps of each system in turn) and print its record")
     ap.add_argument("--forms", default=None,
                     help="forms to run besides --form (comma list); the plain form gives speedup_vs_plain")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -271,6 +274,24 @@ def main():
         n = cfg["n"] if args.n is None else args.n
         log(f"bench: generating {n} bodies")
         pos, vel, mass = bh.generate(kinds[cfg["kind"]], 0, n)
+
+    if args.cpu_only:
+        # the reference's path has no ensembles: each system in turn, rates combined
+        log(f"bench: CPU baseline only, {args.cpu_steps} steps")
+        if systems is None:
+            cpu = cpu_baseline(pos, vel, mass, theta, args.cpu_steps)
+        else:
+            secs, lo = 0.0, 0
+            for ns in systems:
+                r = cpu_baseline(pos[lo:lo + ns], vel[lo:lo + ns], mass[lo:lo + ns], theta, args.cpu_steps)
+                secs += ns * args.cpu_steps / r["value"]
+                lo += ns
+            cpu = {"value": n * args.cpu_steps / secs, "unit": "body-steps/s", "cores": r["cores"],
+                   "kind": r["kind"], "sample": f"{args.cpu_steps} KDK steps of each of the {len(systems)} "
+                                                f"systems ({n} bodies) in turn, {secs:.1f} s"}
+        print(json.dumps({"config": {"workload": cfg["workload"], "name": args.config, "n_bodies": n,
+                                     "theta": theta}, "cpu_baseline": cpu}))
+        return
 
     forms = {"plain": bh.FORM_PLAIN, "mixed": bh.FORM_MIXED, "learned": bh.FORM_LEARNED}
     results = {}
