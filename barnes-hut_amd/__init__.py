@@ -444,7 +444,7 @@ class Engine:
         return t0, t1, w
 
     def group_work(self):
-        """Walk work per group of the last mixed walk: list steps + visits / 2."""
+        """Walk work per group of the last mixed walk: list cost + visits / 2."""
         k = _I64(0)
         self._ck(self.lib.bh_get_groups(self.h, C.byref(k), 0, None, None, None))
         work = np.zeros(k.value, np.uint32)

@@ -174,7 +174,7 @@ struct Bodies {
     double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *m;
     uint64_t* id;
     int8_t* hrank;   // rank in the heavy (direct FP64) set, -1 for light bodies
-    uint32_t* pcost; // walk work of the body's group in the last mixed walk (list steps + visits / 2)
+    uint32_t* pcost; // walk work of the body's group in the last mixed walk (list cost + visits / 2)
 };
 
 struct TreeBuffers {
@@ -282,7 +282,7 @@ struct MixedBuffers {
     uint8_t* gdepth;    // N: tree depth of the group's cells (at the start body)
     uint32_t* g_inter;  // N: interactions evaluated by each group
     uint32_t* g_visit;  // N: nodes visited by each group
-    uint32_t* g_work;   // N: walk work of each group: list-evaluation steps + visits / 2
+    uint32_t* g_work;   // N: walk work of each group: list cost + visits / 2 (the schedules' cost)
     uint64_t* g_t0;     // N: wall clock (100 MHz) when a worker took the group
     uint64_t* g_t1;     // N: wall clock when it finished
     uint32_t* g_worker; // N: worker wave that walked the group
@@ -293,7 +293,7 @@ struct MixedBuffers {
 };
 
 // ---------------------------------------------------- schedule histogram
-// Heaviest-first order: 128 buckets of the estimated walk work c (list steps
+// Heaviest-first order: 128 buckets of the estimated walk work c (list cost
 // + visits / 2), eight per octave, bucket 0 the costliest.  (Simulated with
 // the measured group times of the 300k walk: two buckets per octave of the
 // list length leave the last wave 9% later than eight per octave of the

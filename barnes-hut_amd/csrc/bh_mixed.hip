@@ -418,7 +418,8 @@ __device__ inline void interact(const float4 q, float ox, float oy, float oz, fl
 // Evaluates this lane's share of list[0..ln) for its body (offsets ox, oy,
 // oz): with K lanes per body, the entries sub, sub + K, sub + 2K, ...  FP32
 // FMA inside blocks of 32 such entries, Kahan-compensated across blocks.
-// Returns the wave's steps (entries per lane).
+// Returns the flush's cost for the schedules: the wave's steps (entries per
+// lane).
 template <int K>
 __device__ inline uint32_t flush_list_k(const float4* __restrict__ list, uint32_t ln, uint32_t sub, float ox,
                                         float oy, float oz, float soft2, Acc3& a)
@@ -494,7 +495,10 @@ constexpr uint32_t kMinShare = 32;
 // (body b of stretch s reads lane cnt + b - sE) and added in FP64 into
 // (rx, ry, rz), in list order.  Every body's sum depends on its group and
 // list alone.  All passes share one loop body, which keeps the kernel's code
-// within the instruction cache.  Returns the wave's steps.
+// within the instruction cache.  Returns the flush's cost for the schedules:
+// ln - T, the entries beyond the body lanes' stretches (the wave steps
+// ln - R).  (Chosen by measurement on the 300k learned step: counting the
+// wave's steps ordered the groups 1% worse, the whole list 1.5% worse.)
 __device__ inline uint32_t flush_list_1(const float4* __restrict__ list, uint32_t ln, uint32_t cnt, float ox, float oy,
                                         float oz, float soft2, Acc3& a, double& rx, double& ry, double& rz)
 {
@@ -901,8 +905,9 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 b.vz[i] = b.vz[i] + p.half_dt * az;
             }
         }
-        // the group's work, the schedule's cost unit (a visit measured half
-        // the time of a list step on the 300k walk)
+        // the group's work, the schedules' cost unit: the flushes' costs plus
+        // half the visits (a visit measured half the time of a list step on
+        // the 300k walk)
         const uint32_t work = n_steps + n_visit / 2u;
         if (valid && sub == 0) b.pcost[i] = work;
         if (lane == 0) {

@@ -1,7 +1,6 @@
 // bh_model.hip -- the learned cost model of the LEARNED form: a small MLP
-// that predicts the walk work of every walk group (list-evaluation steps +
-// visited nodes / 2), trained on the device from the work the walk itself
-// records.
+// that predicts the walk work of every walk group (its list cost + visited
+// nodes / 2), trained on the device from the work the walk itself records.
 //
 // Network: 8 FP32 group features -> 32 -> 32 -> 1, ReLU on both hidden
 // layers, weights and biases stored in BF16 (round to nearest even from FP32
@@ -211,7 +210,7 @@ __global__ __launch_bounds__(kNT) void k_model_forward(const float* __restrict__
 
 // ------------------------------------------------------------- samples
 // Appends the groups of this step to the ring buffer: features and target
-// log2(work + 1) / 16, work = the walk's list steps + visits / 2.
+// log2(work + 1) / 16, work = the walk's list cost + visits / 2.
 __global__ __launch_bounds__(kNT) void k_append_samples(const float* __restrict__ feat, TreeBuffers tb,
                                                         MixedBuffers mb, ModelBuffers md)
 {

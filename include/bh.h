@@ -154,7 +154,7 @@ int bh_get_step_phase_ms(bh_ctx *ctx, double *ms5, int64_t *nsteps);
  * depend on it):
  *   KEY_ORDER  dynamic queue in Morton order (default of MIXED)
  *   PREV_COST  heaviest first by the walk work measured in the last step
- *              (list-evaluation steps + visited nodes / 2 per group)
+ *              (list cost + visited nodes / 2 per group, bh_get_group_work)
  *   MODEL      heaviest first by the learned cost model (LEARNED; falls back
  *              to PREV_COST while no model is trained) */
 #define BH_SCHED_KEY_ORDER 0
@@ -198,7 +198,7 @@ int bh_set_moment_cutoff(bh_ctx *ctx, int cutoff);
 /* ------------------------------------------------ learned cost model ---
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
  * accumulation) predicts log2(work + 1) / 16 of every walk group (work =
- * list-evaluation steps + visited nodes / 2, as the walk records it) as
+ * list cost + visited nodes / 2, as the walk records it, bh_get_group_work) as
  * feature 5 (the previous step's value) plus its output; the
  * LEARNED form serves groups heaviest-first by its predictions.  It trains on
  * the device from the work the walk records, during the first
@@ -252,10 +252,12 @@ int bh_get_groups(bh_ctx *ctx, int64_t *n_groups, int64_t capacity, uint32_t *fi
  * worker's index.  Arrays sized by the group count of bh_get_groups. */
 int bh_get_group_timeline(bh_ctx *ctx, int64_t capacity, uint64_t *t_start, uint64_t *t_end, uint32_t *worker);
 
-/* Walk work of each group of the last mixed walk: its list-evaluation steps
- * (list entries per lane) + visited nodes / 2, the cost unit of the
- * heaviest-first schedules and the learned model's target.  Array sized by
- * the group count of bh_get_groups. */
+/* Walk work of each group of the last mixed walk: its list cost + visited
+ * nodes / 2, the cost unit of the heaviest-first schedules and the learned
+ * model's target.  The list cost is its list entries per lane; in a group of
+ * 33-63 bodies, whose idle lanes take over stretches of the list, only the
+ * entries beyond the body lanes' stretches count (the ordering this gives
+ * measured best).  Array sized by the group count of bh_get_groups. */
 int bh_get_group_work(bh_ctx *ctx, int64_t capacity, uint32_t *work);
 
 /* Estimated cost of each group of the last mixed walk, as its heaviest-first
