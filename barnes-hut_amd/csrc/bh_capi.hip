@@ -1044,11 +1044,11 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
     if (c->n == 0) return fail(c, BH_ERR_STATE, "no bodies");
     if (reps < 1) reps = 1;
     const int saved_sched = c->sched;
-    int tform = form;
-    if (form == BH_FORM_LEARNED && !c->model_ready) {
-        tform = BH_FORM_MIXED;
-        c->sched = BH_SCHED_PREV_COST;
-    }
+    // an untrained LEARNED form tunes as MIXED with the context's schedule:
+    // under the previous-cost order the 1M Trojan system's candidates came
+    // within noise of each other and the pick changed from run to run (efac 8
+    // in two of three; the learned walk then ran 2% slower than with efac 2)
+    const int tform = (form == BH_FORM_LEARNED && !c->model_ready) ? BH_FORM_MIXED : form;
     if (c->mixed_blocks_max == 0) c->mixed_blocks_max = bh::mixed_walk_blocks(c->cfg.device);
     hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
     auto time_one = [&](double* ms) -> int {

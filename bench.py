@@ -73,7 +73,7 @@ def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, sys
     eng.set_bodies(pos, vel, mass)
     if systems is not None:
         eng.set_systems(systems)
-    tuned = eng.autotune(reps=2) if tune else None
+    tuned = eng.autotune(reps=3) if tune else None   # a true median of three timings per candidate
     if form == bh.FORM_LEARNED:
         eng.model_configure(train_steps=train_steps)
         eng.step(train_steps, HOUR)
