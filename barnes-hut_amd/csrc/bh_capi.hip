@@ -1071,11 +1071,17 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
     const bh::GroupParams def{};
     bh::GroupParams best = def;
     unsigned best_blocks = c->mixed_blocks_max;
-    double best_ms = 0.0, def_ms = 0.0;
+    double best_ms = 0.0, def_ms = 0.0, best_pick_ms = 0.0;
     int tried = 0, rc = BH_OK;
     const uint32_t gmaxes[3] = {32, 48, 64};
     const double efacs[4] = {1.0, 2.0, 4.0, 8.0};
     const unsigned blocks[2] = {c->mixed_blocks_max, std::max(1u, c->mixed_blocks_max / 2)};
+    struct Cand {
+        bh::GroupParams gp;
+        unsigned blocks;
+        double ms;
+    };
+    std::vector<Cand> cands;
     for (unsigned bl : blocks)
         for (uint32_t gm : gmaxes)
             for (double ef : efacs) {
@@ -1087,12 +1093,28 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
                 if ((rc = time_one(&ms))) break;
                 ++tried;
                 if (gm == def.gmax && ef == def.efac && bl == c->mixed_blocks_max) def_ms = ms;
-                if (tried == 1 || ms < best_ms) {
-                    best_ms = ms;
-                    best = c->gp;
-                    best_blocks = bl;
-                }
+                cands.push_back({c->gp, bl, ms});
+                if (tried == 1 || ms < best_ms) best_ms = ms;
             }
+    // among the candidates within 1% of the fastest (timing noise), the
+    // largest groups, then the full walk grid, then the widest extent limit:
+    // on the 300k system a 48-body limit, and an extent factor of 2 against
+    // 4, timed within noise of the other and then stepped 4% and 0.6% slower
+    bool have = false;
+    for (const Cand& k : cands) {
+        if (k.ms > best_ms * 1.01) continue;
+        const bool better =
+            !have || k.gp.gmax > best.gmax ||
+            (k.gp.gmax == best.gmax &&
+             (k.blocks > best_blocks || (k.blocks == best_blocks && k.gp.efac > best.efac)));
+        if (better) {
+            best = k.gp;
+            best_blocks = k.blocks;
+            best_pick_ms = k.ms;
+            have = true;
+        }
+    }
+    if (have) best_ms = best_pick_ms;
     c->sched = saved_sched;
     c->gp = best;
     c->mixed_blocks = best_blocks;

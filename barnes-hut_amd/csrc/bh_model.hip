@@ -19,6 +19,7 @@
 //   W1[32][8] | b1[32] | W2[32][32] | b2[32] | W3[32] | b3
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "bh_internal.hpp"
@@ -396,7 +397,11 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
                            uint32_t max_groups, const uint16_t* model, hipStream_t s)
 {
-    k_group_features<<<512, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups, model);
+    // one wavefront per group, grid-stride: 512 blocks up to about 300k bodies
+    // (more blocks there only add weight loads: +1% on the 300k step), more
+    // beyond so the groups' load latencies overlap (-1% at 1M with 1,750)
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>(std::max<size_t>(n / 600, 512), 4096));
+    k_group_features<<<blocks, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups, model);
 }
 
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s)
