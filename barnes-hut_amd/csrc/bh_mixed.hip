@@ -32,8 +32,8 @@
 // FP32 FMA over blocks of 32 entries, each block sum added into a
 // Kahan-compensated FP32 accumulator; lanes past a group's bodies take over
 // stretches of the bodies' lists.  A group's own bodies appear in the
-// list with offsets bit-identical to the lane's own, so r^2 = 0 marks (and
-// drops) the self term.
+// list with offsets bit-identical to the lane's own, so the self term has
+// dx = dy = dz = 0 and adds +0 (r^2 starts from a small positive floor).
 //
 // Heavy bodies (the central mass and planets: the heaviest bodies, up to 16)
 // are not in the light moments; every light body adds them by an FP64 direct
@@ -397,19 +397,30 @@ struct Acc3 {
     }
 };
 
-__device__ inline void interact(const float4 q, float ox, float oy, float oz, float soft2, float& bx, float& by,
+// The r^2 every list interaction starts from: the softening, but at least
+// 1e-24 x max(1, (G M)^(2/3)) for the light mass G M of the group's system.
+// The self entry (the lane's own body: dx = dy = dz = 0 exactly) then has
+// rsq <= 1e12 / max(1, (G M)^(1/3)) and G m rsq^3 <= 1e36, so it adds
+// 0 * finite = +0 with no clamp in the loop.  Another pair's r^2 moves by at
+// most the floor, below half an ulp once r^2 exceeds 2^25 x the floor
+// (a distance of 6e-9 in a system with G M <= 1).
+__device__ inline float list_r2_floor(float gm, float soft2)
+{
+    const float e = gm > 1.f ? exp2f((2.f / 3.f) * log2f(gm)) : 1.f;
+    return fmaxf(soft2, 1e-24f * e);
+}
+
+// r2_0 is list_r2_floor's value: with it every r2 is positive and every
+// rsq finite
+__device__ inline void interact(const float4 q, float ox, float oy, float oz, float r2_0, float& bx, float& by,
                                 float& bz)
 {
     const float dx = q.x - ox;
     const float dy = q.y - oy;
     const float dz = q.z - oz;
-    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
-    // The self entry (the lane's own body: dx = dy = dz = 0 exactly) has
-    // r2 = 0 and rsq = +inf; clamping s to 1e30 -- far above any physical
-    // G m / r^3 -- keeps it finite, so it adds 0 * 1e30 = +0, and it costs one
-    // instruction where a compare and a select cost two.
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, r2_0)));
     const float inv = __builtin_amdgcn_rsqf(r2);
-    const float s = fminf(q.w * (inv * inv * inv), 1e30f);
+    const float s = q.w * (inv * inv * inv);
     bx = fmaf(s, dx, bx);
     by = fmaf(s, dy, by);
     bz = fmaf(s, dz, bz);
@@ -723,6 +734,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
         uint32_t ln = 0, sp = 1, n_inter = 0, n_visit = 0, n_steps = 0;
         bool overflow = false;
         const uint32_t gsys = tb.bsys[b0];
+        const float r2_0 = list_r2_floor(mb.node[gsys].gm, soft2f);   // the system's root
         // heavy bodies of the system: FP64 direct sum in rank order
         double hax = 0.0, hay = 0.0, haz = 0.0;
         const SysScalars* sy = tb.sys + gsys;
@@ -836,7 +848,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const uint32_t na = static_cast<uint32_t>(__popcll(am));
                 if (ln + na > static_cast<uint32_t>(kListCap)) {
                     __builtin_amdgcn_wave_barrier();
-                    n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, rx, ry, rz);
+                    n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, r2_0, acc, rx, ry, rz);
                     __builtin_amdgcn_wave_barrier();
                     ln = 0;
                 }
@@ -872,7 +884,7 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
         }
         __builtin_amdgcn_wave_barrier();
-        n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, soft2f, acc, rx, ry, rz);
+        n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, r2_0, acc, rx, ry, rz);
         __builtin_amdgcn_wave_barrier();
         // the K partial sums of a body, added in FP64 in sub order (a Kahan
         // sum's value is s - c: c holds the negative of the rounding lost)

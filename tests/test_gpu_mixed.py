@@ -93,6 +93,31 @@ def test_mixed_edge_cases(bh, orc, engine):
         assert np.all(np.isfinite(acc1)), name
 
 
+@pytest.mark.parametrize("scale", [1.0, 1.5e11])
+def test_mixed_si_units(bh, orc, engine, scale):
+    """SI G with solar and asteroid masses (G M up to 1.3e20), on a unit box
+    and at 1.5e11 m, with co-located pairs: the list's r^2 floor grows with
+    (G M)^(2/3), so the self and co-located terms stay finite (+0) for any
+    units and distinct pairs keep FP32 accuracy (theta = 0 against the FP64
+    direct sum)."""
+    G_si = 6.674e-11
+    rng = np.random.default_rng(11)
+    p = rng.uniform(-1, 1, (600, 3)) * scale
+    p[100:110] = p[90:100]   # ten co-located pairs
+    mm = rng.uniform(1e18, 1e22, 600)
+    mm[0] = 1.989e30
+    say("mixed SI", scale)
+    engine.set_bodies(p, np.zeros_like(p), mm)
+    acc = engine.compute_accelerations(form=bh.FORM_MIXED, theta=0.0, G_=G_si)
+    st = engine.get_bodies()
+    assert np.all(np.isfinite(acc))
+    ds = orc.direct(st["pos"], st["mass"], np.arange(len(mm)), G_=G_si)
+    err = np.linalg.norm(acc - ds, axis=1) / np.linalg.norm(ds, axis=1).max()
+    assert err.max() < 1e-4, err.max()
+    acc1 = engine.compute_accelerations(form=bh.FORM_MIXED, theta=1.05, G_=G_si)
+    assert np.all(np.isfinite(acc1))
+
+
 def _by_id(st):
     o = np.argsort(st["ids"])
     return st["pos"][o], st["vel"][o]
