@@ -118,6 +118,21 @@ def test_mixed_si_units(bh, orc, engine, scale):
     assert np.all(np.isfinite(acc1))
 
 
+def test_mixed_softening(bh, orc, engine):
+    """soft2 > 0 replaces the list's r^2 floor when larger: theta = 0 matches
+    the softened FP64 direct sum, and the softening changes the result."""
+    say("mixed softened belt 4000")
+    pos, vel, m = bh.generate(bh.GEN_BELT, 2, 4000)
+    engine.set_bodies(pos, vel, m)
+    acc = engine.compute_accelerations(form=bh.FORM_MIXED, theta=0.0, soft2=1e-6)
+    st = engine.get_bodies()
+    ds = orc.direct(st["pos"], st["mass"], np.arange(len(m)), soft2=1e-6)
+    err = np.linalg.norm(acc - ds, axis=1) / np.linalg.norm(ds, axis=1)
+    assert err.max() < 1e-5, err.max()
+    d0 = orc.direct(st["pos"], st["mass"], np.arange(len(m)))
+    assert np.abs(ds - d0).max() > 0
+
+
 def _by_id(st):
     o = np.argsort(st["ids"])
     return st["pos"][o], st["vel"][o]

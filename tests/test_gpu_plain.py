@@ -166,6 +166,21 @@ def test_full_size_300k(bh, orc, engine):
     assert g["mass"].sum() == pytest.approx(m.sum(), rel=1e-15)
 
 
+def test_softening_vs_oracle(bh, orc, engine):
+    """soft2 > 0, which the reference adds to every r^2 (src/traversal.cpp:75):
+    bit-identical to the oracle, and different from soft2 = 0."""
+    say("plain softened belt 4000")
+    pos, vel, m = bh.generate(bh.GEN_BELT, 2, 4000)
+    o = orc.force_path(pos, m, vel=vel, theta=1.05, soft2=1e-6)
+    engine.set_bodies(pos, vel, m)
+    acc = engine.compute_accelerations(theta=1.05, soft2=1e-6)
+    st = engine.get_bodies()
+    assert np.array_equal(st["ids"], o["ids"])
+    assert np.array_equal(bits(acc), bits(o["acc"]))
+    acc0 = engine.compute_accelerations(theta=1.05)
+    assert not np.array_equal(bits(acc0), bits(acc))
+
+
 def test_plain_error_vs_direct_sum(bh, orc, engine):
     say("belt 19054 error vs direct sum")
     pos, vel, m = bh.generate(bh.GEN_BELT, 0, 19054)

@@ -183,6 +183,25 @@ def test_walk_two_body_exact(orc):
         assert np.allclose(r["acc"][i], s * (sp[j] - sp[i]), rtol=1e-14)
 
 
+def test_walk_two_body_softened(orc):
+    """soft2 is added to r^2 once (src/traversal.cpp:75): with eps^2 = 0.25
+    the two-body force is G m d / (|d|^2 + eps^2)^(3/2), and the direct sum
+    agrees to rounding."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    mass = np.array([2.0e30, 3.0e24])
+    eps2 = 0.25
+    r = orc.force_path(pos, mass, theta=0.5, soft2=eps2)
+    sp, sm = r["pos"], r["mass"]
+    for i in range(2):
+        j = 1 - i
+        d = sp[j] - sp[i]
+        want = G * sm[j] * d / (d @ d + eps2) ** 1.5
+        assert np.allclose(r["acc"][i], want, rtol=1e-14, atol=0)
+        assert not np.allclose(r["acc"][i], G * sm[j] * d / (d @ d) ** 1.5, rtol=1e-3, atol=0)
+    ds = orc.direct(sp, sm, np.arange(2), soft2=eps2)
+    assert np.allclose(ds, r["acc"], rtol=1e-14, atol=0)
+
+
 def test_direct_three_body_line(orc):
     pos = np.array([[0.0, 0, 0], [1.0, 0, 0], [3.0, 0, 0]])
     mass = np.array([1e30, 2e30, 4e30])
@@ -270,6 +289,17 @@ def test_oracle_matches_live_reference(orc, bh, seed, n, theta):
     assert np.array_equal(a["keys"], b["keys"]) and np.array_equal(a["perm"], b["perm"])
     assert tree_digest(*a["tree"]) == tree_digest(*b["tree"])
     assert np.array_equal(bits(a["acc"]), bits(b["acc"]))
+
+
+def test_oracle_matches_live_reference_softened(orc, bh):
+    if not orc.ref_available("strict"):
+        pytest.skip("oracle/_ref not built here (needs the reference checkout)")
+    pos, vel, m = bh.generate(bh.GEN_BELT, 14, 2000)
+    a = orc.force_path(pos, m, theta=1.05, soft2=1e-6)
+    b = orc.ref_force_path(pos, m, theta=1.05, soft2=1e-6)
+    assert np.array_equal(bits(a["acc"]), bits(b["acc"]))
+    c = orc.force_path(pos, m, theta=1.05)
+    assert not np.array_equal(bits(a["acc"]), bits(c["acc"]))
 
 
 def test_oracle_counts_are_plausible(orc, bh):
