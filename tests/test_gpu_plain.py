@@ -166,14 +166,17 @@ def test_full_size_300k(bh, orc, engine):
     assert g["mass"].sum() == pytest.approx(m.sum(), rel=1e-15)
 
 
-def test_softening_vs_oracle(bh, orc, engine):
-    """soft2 > 0, which the reference adds to every r^2 (src/traversal.cpp:75):
-    bit-identical to the oracle, and different from soft2 = 0."""
-    say("plain softened belt 4000")
+@pytest.mark.parametrize("G_,soft2", [(None, 1e-6), (6.674e-11, 0.0), (6.674e-11, 1e-4)])
+def test_softening_and_G_vs_oracle(bh, orc, engine, G_, soft2):
+    """soft2 > 0, which the reference adds to every r^2 (src/traversal.cpp:75),
+    and a G other than the default (the reference multiplies G m by 1/r^3):
+    bit-identical to the oracle, and different from the default run."""
+    G_ = bh.G if G_ is None else G_
+    say("plain belt 4000, G", G_, "soft2", soft2)
     pos, vel, m = bh.generate(bh.GEN_BELT, 2, 4000)
-    o = orc.force_path(pos, m, vel=vel, theta=1.05, soft2=1e-6)
+    o = orc.force_path(pos, m, vel=vel, theta=1.05, G_=G_, soft2=soft2)
     engine.set_bodies(pos, vel, m)
-    acc = engine.compute_accelerations(theta=1.05, soft2=1e-6)
+    acc = engine.compute_accelerations(theta=1.05, G_=G_, soft2=soft2)
     st = engine.get_bodies()
     assert np.array_equal(st["ids"], o["ids"])
     assert np.array_equal(bits(acc), bits(o["acc"]))
