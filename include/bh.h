@@ -64,7 +64,9 @@ typedef struct bh_config {
     int form;         /* bh_form used by bh_step */
     double theta;     /* opening angle (--theta, default 1.05) */
     double G;         /* gravitational constant (BH_G) */
-    double soft2;     /* softening eps^2 added to r^2 (0.0 in the reference) */
+    double soft2;     /* softening eps^2 added to r^2 (0.0 in the reference); the
+                         mixed forms' FP32 list terms use at least
+                         1e-24 * max(1, (G M)^(2/3)), G M the system's light mass */
 } bh_config;
 
 typedef struct bh_ctx bh_ctx;
