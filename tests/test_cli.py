@@ -2,6 +2,7 @@
 force policies and bad options exit before any device call."""
 import pathlib
 import subprocess
+import sys
 
 import pytest
 
@@ -32,3 +33,13 @@ def test_bad_options():
     assert run("--form", "fast", "-f", "x.csv").returncode == 2
     assert run("--dt", "3w", "-f", "x.csv").returncode == 1
     assert run().returncode == 1   # no input file
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py runs on one GPU: --gpus above 1 exits with status 2 and says so,
+    before touching any device."""
+    root = pathlib.Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
