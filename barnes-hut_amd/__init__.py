@@ -78,6 +78,7 @@ SIGNATURES = [
     ("bh_export_topology", C.c_int, [_P, _P, _P, _P, _P]),
     ("bh_set_counting", C.c_int, [_P, C.c_int]),
     ("bh_get_counters", C.c_int, [_P, _P, _P, _P]),
+    ("bh_debug_raise_error", C.c_int, [_P, C.c_uint32]),
     ("bh_get_phase_ms", C.c_int, [_P, _P]),
     ("bh_get_step_phase_ms", C.c_int, [_P, _P, C.POINTER(_I64)]),
     ("bh_direct_sum", C.c_int, [_P, _I64, _P, _D, _D, _P]),
@@ -303,6 +304,10 @@ class Engine:
         t = np.zeros(self.n, np.int64)
         self._ck(self.lib.bh_get_counters(self.h, _ptr(a), _ptr(o), _ptr(t)))
         return a, o, t
+
+    def debug_raise_error(self, bits: int):
+        """Test hook: raise device guard bits on the context stream."""
+        self._ck(self.lib.bh_debug_raise_error(self.h, bits))
 
     def phase_ms(self) -> np.ndarray:
         out = np.zeros(5)

@@ -79,6 +79,7 @@ struct bh_ctx {
     bool have_counts = false;
     bool have_groups = false;
     bool prev_counts = false;         // the bodies carry the walk work of an earlier mixed walk
+    bool evaluated = false;           // a force evaluation has permuted the bodies since bh_set_bodies
     hipEvent_t ev[5] = {};
     // per-step phase events of bh_step: ring of kRing sets of 5
     std::vector<hipEvent_t> ring;
@@ -224,6 +225,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     bh::launch_permute(c->cur, c->alt, c->tb, c->n, s);
     std::swap(c->cur, c->alt);
     c->parity ^= 1;
+    c->evaluated = true;
     MARK(2);
     TreeBuffers tb = c->tb;
     if (!mixed) {
@@ -363,11 +365,18 @@ int model_alloc(bh_ctx* c)
     return BH_OK;
 }
 
-// Error bits raised by device-side guards during the last evaluation.
+// Error bits raised by device-side guards since the last check.  No kernel
+// clears them, so a bit raised in any step of a multi-step call is still
+// set when the call ends; the check reads them and clears them for the next
+// call.
 int check_device_errors(bh_ctx* c)
 {
     uint32_t err = 0;
     HIP_TRY(c, hipMemcpy(&err, &c->tb.sc->error, sizeof err, hipMemcpyDeviceToHost));
+    if (err) {
+        const uint32_t zero = 0;
+        HIP_TRY(c, hipMemcpy(&c->tb.sc->error, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
     if (err & bh::kErrStack) return fail(c, BH_ERR_STATE, "mixed walk: traversal stack capacity exceeded");
     if (err) return fail(c, BH_ERR_STATE, "device guard raised error bits 0x%x", err);
     uint32_t chain = 0;
@@ -650,6 +659,7 @@ int bh_set_bodies(bh_ctx* c, int64_t n, const bh_vec3* pos, const bh_vec3* vel, 
     c->have_counts = false;
     c->have_groups = false;
     c->prev_counts = false;
+    c->evaluated = false;
     if (n == 0) {
         c->n_sys = 1;
         c->tb.n_sys = 1;
@@ -903,6 +913,25 @@ int bh_get_counters(bh_ctx* c, int64_t* accepted, int64_t* opened, int64_t* test
 }
 
 namespace {
+// ORs bits into the step's guard word on the context stream, where a
+// device-side guard of a walk would raise them.
+__global__ void k_raise_error(StepScalars* sc, uint32_t bits)
+{
+    if (threadIdx.x == 0) atomicOr(&sc->error, bits);
+}
+}  // namespace
+
+int bh_debug_raise_error(bh_ctx* c, uint32_t bits)
+{
+    if (check_ctx(c)) return BH_ERR_ARG;
+    c->err.clear();
+    if (!c->tb.sc) return fail(c, BH_ERR_STATE, "no bodies (bh_set_bodies first)");
+    k_raise_error<<<1, 64, 0, c->stream>>>(c->tb.sc, bits);
+    HIP_TRY(c, hipGetLastError());
+    return BH_OK;
+}
+
+namespace {
 int phase_ms(bh_ctx* c, const hipEvent_t* ev, double* ms5)
 {
     float f;
@@ -973,6 +1002,11 @@ int bh_set_systems(bh_ctx* c, int64_t n_systems, const int64_t* counts)
     if (n_systems < 1 || n_systems > bh::kMaxSystems || !counts)
         return fail(c, BH_ERR_ARG, "1..%d systems", bh::kMaxSystems);
     if (c->n == 0) return fail(c, BH_ERR_STATE, "no bodies (bh_set_bodies first)");
+    // a force evaluation sorts the bodies over the whole index range while
+    // they form one system, so fixed ranges would no longer hold the systems
+    // the caller concatenated
+    if (c->evaluated)
+        return fail(c, BH_ERR_STATE, "bh_set_systems must follow bh_set_bodies before any force evaluation");
     int64_t sum = 0;
     for (int64_t k = 0; k < n_systems; ++k) {
         if (counts[k] < 1) return fail(c, BH_ERR_ARG, "system %lld has %lld bodies", (long long)k, (long long)counts[k]);

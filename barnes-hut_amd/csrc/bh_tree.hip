@@ -220,8 +220,9 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
             }
         }
     }
+    // sc->error is not cleared here: the guard bits stay raised across the
+    // steps of one call until the host reads and clears them
     if (threadIdx.x == 0) {
-        sc->error = 0;
         sc->group_counter = 0;
         sc->block_counter = 0;
         sc->heavy_counter = 0;

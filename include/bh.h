@@ -144,6 +144,13 @@ int bh_export_topology(bh_ctx *ctx, uint64_t *prefix, uint8_t *depth, double *ma
  * output pointer may be NULL. */
 int bh_set_counting(bh_ctx *ctx, int on);
 int bh_get_counters(bh_ctx *ctx, int64_t *accepted, int64_t *opened, int64_t *tested);
+/* Device-side guards (walk index range, mixed-walk stack capacity) raise
+ * bits in a per-context word that no kernel clears: a bit raised in any step
+ * of bh_step is reported when the call returns (BH_ERR_STATE), and the check
+ * clears it.  This test hook raises `bits` (nonzero) on the context stream as
+ * a guard would, so that the reporting path can be tested without corrupt
+ * input.  No counterpart in the reference. */
+int bh_debug_raise_error(bh_ctx *ctx, uint32_t bits);
 /* Device time in ms of the phases of the last force evaluation:
  * [0] kick+drift+box+keys, [1] sort+permute, [2] tree build, [3] walk(+kick),
  * [4] total. */
@@ -284,7 +291,9 @@ int bh_direct_sum(bh_ctx *ctx, int64_t m, const int64_t *idx, double G, double s
  * Every system gets its own box, keys, tree, walk groups and heavy set, so
  * its forces are bit-identical to a run of that system alone.  The MIXED and
  * LEARNED forms run ensembles; PLAIN returns BH_ERR_UNSUPPORTED for S > 1.
- * bh_set_bodies resets to one system. */
+ * bh_set_bodies resets to one system.  bh_set_systems must come after
+ * bh_set_bodies and before any force evaluation (BH_ERR_STATE otherwise): an
+ * evaluation sorts the bodies of a single system over the whole range. */
 #define BH_MAX_SYSTEMS 256
 int bh_set_systems(bh_ctx *ctx, int64_t n_systems, const int64_t *counts);
 /* *n_systems gets S; offsets (S + 1 entries, may be NULL) the body ranges. */
