@@ -43,3 +43,24 @@ def test_bench_refuses_several_gpus():
                        timeout=120, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+
+
+def test_integration_sketch_is_the_compiled_dropin():
+    """The per-call loop printed in INTEGRATION.md is the part of
+    barnes-hut_amd/host/bh_dropin.cpp between its sketch markers, verbatim (up
+    to the first indent), so the documented integration is the one that
+    compiles and that the GPU tests run."""
+    root = pathlib.Path(__file__).resolve().parents[1]
+    src = (root / "barnes-hut_amd" / "host" / "bh_dropin.cpp").read_text()
+    body = src.split("// [sketch begin]\n", 1)[1].split("    // [sketch end]", 1)[0]
+    want = [ln[4:] if ln.startswith("    ") else ln for ln in body.rstrip().splitlines()]
+    doc = (root / "INTEGRATION.md").read_text()
+    sect = doc.split("## 1. Drop-in", 1)[1].split("## 2.", 1)[0]
+    block = sect.split("```cpp\n", 1)[1].split("```", 1)[0]
+    got = block.rstrip().splitlines()
+    assert got[:2] == ['#include "bh.h"', ""]
+    assert got[2:] == want
+    # every call the loop makes per step, in the order that keeps the host
+    # arrays in the engine's Morton order
+    loop = "\n".join(want)
+    assert loop.index("bh_set_bodies") < loop.index("bh_compute_accelerations") < loop.index("bh_get_bodies")
