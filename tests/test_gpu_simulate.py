@@ -24,6 +24,27 @@ def run(cwd, *args):
     return r.stdout
 
 
+def read_vtp(path):
+    """Arrays of one snapshot by name: floats parsed from their 17-digit text
+    (exact FP64), integers as int64 / uint64, vectors as (n, 3)."""
+    piece = ET.parse(path).getroot().find("PolyData").find("Piece")
+    out = {}
+    arrays = list(piece.find("Points").findall("DataArray")) + list(piece.find("PointData").findall("DataArray")) + \
+        list(piece.find("Verts").findall("DataArray"))
+    for da in arrays:
+        words = da.text.split()
+        typ = da.get("type")
+        if typ == "Float64":
+            v = np.array([float(w) for w in words])
+        elif typ == "UInt64":
+            v = np.array([int(w) for w in words], np.uint64)
+        else:
+            v = np.array([int(w) for w in words], np.int64)
+        comps = int(da.get("NumberOfComponents", "1"))
+        out[da.get("Name")] = v.reshape(-1, 3) if comps == 3 else v
+    return out
+
+
 def field(out, name):
     line = [ln for ln in out.splitlines() if ln.startswith(name)][-1]
     return float(line.split()[-1])
@@ -65,6 +86,28 @@ def test_simulate_reference_run_matches_oracle(bh, orc, tmp_path):
     assert piece.get("NumberOfPoints") == str(n)
     names = [d.get("Name") for d in piece.find("PointData").findall("DataArray")]
     assert names == ["rank", "body_id", "velocity", "mass", "acceleration"]
+    # frame contents (src/io.cpp:84-232): every frame's positions, ids,
+    # velocities, masses and |a| equal the oracle's state after the step that
+    # wrote it, bit for bit (17 significant digits round-trip FP64)
+    snap_steps, t, nxt, k = [], 0.0, 0.0, 0
+    while t < 10 / 24:
+        if t >= nxt or k == 0:
+            snap_steps.append(k)
+            nxt += 3 / 24
+        k, t = k + 1, t + 1 / 24
+    for f, k in enumerate(snap_steps):
+        say("frame", f, "after step", k)
+        fr = read_vtp(tmp_path / "snap" / "0" / f"sim.{f}.vtp")
+        o = orc.simulate(rpos, rvel, rm, k + 1, 1 / 24, ids=rid)
+        assert np.array_equal(fr["body_id"], o["ids"])
+        assert np.array_equal(fr["rank"], np.zeros(n, np.int64))
+        assert np.array_equal(fr["position"].view(np.uint64), o["pos"].view(np.uint64))
+        assert np.array_equal(fr["velocity"].view(np.uint64), o["vel"].view(np.uint64))
+        assert np.array_equal(fr["mass"].view(np.uint64), o["mass"].view(np.uint64))
+        a = o["acc"]
+        amag = np.sqrt(a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1] + a[:, 2] * a[:, 2])
+        assert np.array_equal(fr["acceleration"].view(np.uint64), amag.view(np.uint64))
+        assert np.array_equal(fr["offsets"], np.arange(1, n + 1)) and np.array_equal(fr["connectivity"], np.arange(n))
     # a second, normal run compares against the reference file
     out2 = run(tmp_path, "-f", str(csv), "--dt", "1h", "--tend", "10h", "--vs", "100d", "-o", "snap2")
     assert field(out2, "SUMMED_DIST_ERROR") == 0.0
