@@ -13,14 +13,17 @@ graph.
 Prints one JSON line.  `value` is body-steps/s of the form given by --form
 (default: LEARNED, the mixed-precision walk with the learned schedule, whose
 cost model trains on the first 50 steps before the timed region), with the
-plain FP64 walk (bit-identical to the reference) and the mixed walk without
-the cost model reported beside it; the maximum relative force error of each
-form against an FP64 direct sum on 4,096 seeded sample bodies plus the Sun
-and planets; the roofline of the dominant walk kernel (algorithmic flops per
-launch / its mean duration, HIP events on the library stream over eager steps
-that follow the timed region); and a CPU baseline: the reference's own code
-(oracle/_ref, Release-like flags, OpenMP) on a bounded sample of the same
-workload.
+plain FP64 walk (bit-identical to the reference), the mixed walk in key order
+and the mixed walk with the schedule the learned one replaces (heaviest first
+by the previous step's work, on the same states: gain_vs_prev_cost) reported
+beside it; the maximum relative force error of each form against an FP64
+direct sum on 4,096 seeded sample bodies plus the Sun and planets; the
+roofline of the dominant walk kernel (algorithmic flops per launch, W = 21 I
++ 9 O_c from the plain walk's counters on the same state, / its mean
+duration, HIP events on the library stream over eager steps that follow the
+timed region); and a CPU baseline: the reference's own code (oracle/_ref,
+Release-like flags, OpenMP) on a bounded sample of the same workload, timed
+after one warm-up step.
 
 Other configurations (--config): c0 (19,054 bodies, plain), c1-theta0.5,
 c2 (4,194,304 bodies with a Kuiper belt, theta 0.7), c3 (1,048,576 bodies
@@ -64,19 +67,28 @@ def log(*a):
 PHASE_STEPS = 5
 
 
-def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, systems=None, tune=False):
+def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, systems=None, tune=False,
+             sched=None, shape=None, pre_steps=0):
     """Fresh engine; the LEARNED form first trains its cost model on the
     first `train_steps` steps (untimed), then every form runs `warmup`
     untimed and `steps` timed steps (graph replay).  PHASE_STEPS eager steps
-    after the timed region give the per-phase device times."""
+    after the timed region give the per-phase device times.  sched / shape
+    (a tuned group shape) / pre_steps (untimed steps before the warmup) let a
+    MIXED run time another schedule on the states a LEARNED run times."""
     eng = bh.Engine(form=form, theta=theta)
     eng.set_bodies(pos, vel, mass)
     if systems is not None:
         eng.set_systems(systems)
+    if sched is not None:
+        eng.set_schedule(sched)
+    if shape is not None:
+        eng.set_group_params(shape["gmax"], shape["efac"], shape["occ"], shape["walk_blocks"])
     tuned = eng.autotune(reps=3) if tune else None   # a true median of three timings per candidate
     if form == bh.FORM_LEARNED:
         eng.model_configure(train_steps=train_steps)
         eng.step(train_steps, HOUR)
+    if pre_steps:
+        eng.step(pre_steps, HOUR)
     eng.step(warmup, HOUR)
     r0 = eng.graph_replays()
     t0 = time.perf_counter()
@@ -131,33 +143,97 @@ def ensemble_error(bh, eng, sizes, samples, seed=0):
                 includes_ids_0_8=True, per_system=per)
 
 
-def plain_walk_work(bh, eng, theta):
-    """Algorithmic flops of one plain walk launch: 9 per tested node (3 sub,
-    5 for r^2 + soft2, 1 for theta^2 r^2) + 12 per accepted node (sqrt, div,
-    2 mul for 1/r^3, 2 mul for G m / r^3, 3 mul + 3 add)."""
+def algorithmic_work(bh, eng, theta):
+    """Algorithmic flops of one walk over the engine's current state, the
+    same for every form: W = 21 I + 9 O_c per body, I = nodes the reference's
+    criterion accepts (src/traversal.cpp:63-98), O_c = the other compressed
+    nodes it tests (opened branching nodes and the body's own leaf).  A
+    tested node costs 9 (3 sub, 5 for r^2 + soft2, 1 for theta^2 r^2), an
+    accepted one 12 more (sqrt, div, 2 mul for 1/r^3, 2 mul for G m / r^3,
+    3 mul + 3 add).  I and O_c come from the plain walk's counters
+    (bh_get_counters) in one counted PLAIN evaluation of the same state.  The
+    compressed node count gives the walk's compulsory bytes (each 48-byte
+    node record read once, and per body 24 B position + 8 B mass + 24 B
+    velocity read, 24 B velocity + 24 B acceleration written)."""
     eng.set_counting(True)
     eng.compute_accelerations(form=bh.FORM_PLAIN, theta=theta)
     acc, opened, tested = eng.counters()
     eng.set_counting(False)
-    flops = 9.0 * tested.sum() + 12.0 * acc.sum()
-    return flops, float(acc.mean()), float(tested.mean()), float(opened.mean())
+    n_full, n_comp = eng.tree_size()
+    I = float(acc.sum())
+    Oc = float(tested.sum()) - I
+    n = len(acc)
+    return dict(flops_per_launch=21.0 * I + 9.0 * Oc, accepted_per_body=I / n, opened_compressed_per_body=Oc / n,
+                opened_full_chain_per_body=float(opened.mean()), tested_per_body=float(tested.mean()),
+                compressed_nodes=int(n_comp), compulsory_bytes=48.0 * n_comp + 104.0 * n)
+
+
+def algorithmic_work_any(bh, eng, theta, systems):
+    """algorithmic_work of the engine's state; an ensemble (which the plain
+    form does not run) is counted system by system on engines of its own."""
+    if systems is None:
+        return algorithmic_work(bh, eng, theta)
+    st = eng.get_bodies()
+    off = eng.systems()
+    tot = None
+    for k in range(len(systems)):
+        sl = slice(int(off[k]), int(off[k + 1]))
+        with bh.Engine(form=bh.FORM_PLAIN, theta=theta) as e1:
+            e1.set_bodies(st["pos"][sl], st["vel"][sl], st["mass"][sl])
+            w = algorithmic_work(bh, e1, theta)
+        ns = sl.stop - sl.start
+        if tot is None:
+            tot = {k_: 0.0 for k_ in w}
+        for k_, v in w.items():
+            tot[k_] += v * ns if k_.endswith("_per_body") else v
+    n = len(st["mass"])
+    for k_ in tot:
+        if k_.endswith("_per_body"):
+            tot[k_] /= n
+    tot["compressed_nodes"] = int(tot["compressed_nodes"])
+    return tot
 
 
 def mixed_walk_work(eng):
-    """Algorithmic flops of one mixed walk launch from its last groups:
-    every body of a group evaluates the group's list, 20 FP32 flops per entry
-    (3 sub, 3 FMA for r^2, rsqrt, 2 mul for 1/r^3, 1 mul by G m, 3 FMA), and
-    every group tests each visited node once, 12 FP64 flops (3 sub, 3 abs-sub,
-    3 max, 5 for d^2, 1 mul; counted as 12)."""
+    """Flops the mixed walk executes in one launch, from its last groups:
+    every body of a group evaluates the group's whole list, 20 FP32 flops per
+    entry (3 sub, 3 FMA for r^2, rsqrt, 2 mul for 1/r^3, 1 mul by G m,
+    3 FMA), and every group tests each visited node once, 12 FP64 flops
+    (3 sub, 3 abs-sub, 3 max, 5 for d^2, 1 mul; counted as 12).  A group's
+    list holds what the least-accepting body of the group needs, so this
+    exceeds the algorithmic work of the reference's per-body criterion."""
     first, inter, visit = eng.groups()
     cnt = np.diff(first.astype(np.int64))
     flops = 20.0 * float(np.sum(cnt * inter.astype(np.float64))) + 12.0 * float(visit.sum())
     return flops, dict(groups=int(len(inter)), bodies_per_group=float(cnt.mean()),
                        list_per_group=float(inter.mean()), visited_per_group=float(visit.mean()),
+                       list_entries_per_body=float(np.sum(cnt * inter.astype(np.float64)) / cnt.sum()),
                        lane_utilisation=float(cnt.sum() / (64.0 * len(inter))))
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu), or the platform's processor string."""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(pos, vel, mass, theta, steps):
+    """The reference's force path on the host cores: one untimed warm-up KDK
+    step from t = 0 (the first, unsorted build), then `steps` timed steps from
+    that state (src/main.cpp:185,323,331-334 time the step loop the same way,
+    wall clock around the steps).  It runs the reference's own sources
+    (src/morton_keys.cpp, linear_octree.cpp, traversal.cpp) as built in
+    oracle/_ref with Release-like flags; the built library travels with the
+    tree, the sources do not.  Without it the oracle's C restatement runs
+    (kind "port")."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
@@ -168,19 +244,26 @@ def cpu_baseline(pos, vel, mass, theta, steps):
         if oracle.ref_available(k):
             kind = k
             break
-    t0 = time.perf_counter()
     if kind:
-        oracle.ref_simulate(pos, vel, mass, steps, HOUR, theta=theta, kind=kind)
+        run = lambda p, v, m, k_, t0=0.0, acc=None: oracle.ref_simulate(p, v, m, k_, HOUR, theta=theta, kind=kind,
+                                                                         t0=t0, acc=acc)
         what = f"reference sources (src/morton_keys.cpp, linear_octree.cpp, traversal.cpp) built {kind}"
         k = "reference"
     else:
-        oracle.simulate(pos, vel, mass, steps, HOUR, theta=theta)
+        run = lambda p, v, m, k_, t0=0.0, acc=None: oracle.simulate(p, v, m, k_, HOUR, theta=theta, t0=t0, acc=acc)
         what = "C restatement oracle/bh_oracle.c"
         k = "port"
+    w0 = time.perf_counter()
+    st = run(pos, vel, mass, 1)
+    w1 = time.perf_counter()
+    t0 = time.perf_counter()
+    run(st["pos"], st["vel"], st["mass"], steps, t0=HOUR, acc=st["acc"])
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": cores, "kind": k,
-            "sample": f"{steps} KDK steps of the {n}-body workload (theta {theta}, dt 1 h) from t=0, "
-                      f"{what}, OpenMP threads {cores}, {dt:.1f} s"}
+    model = cpu_model()
+    return {"value": n * steps / dt, "unit": "body-steps/s", "cores": cores, "kind": k, "cpu_model": model,
+            "sample": f"{steps} timed KDK steps of the {n}-body workload (theta {theta}, dt 1 h) after one untimed "
+                      f"warm-up step from t=0 ({w1 - w0:.1f} s), {what}, OpenMP threads {cores} on {model}, "
+                      f"{dt:.1f} s"}
 
 
 CONFIGS = {
@@ -226,6 +309,8 @@ def main():
     ap.add_argument("--samples", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-prev-cost", action="store_true",
+                    help="skip the MIXED run with the previous-step schedule beside LEARNED")
     ap.add_argument("--cpu-only", action="store_true",
                     help="only time the CPU baseline of the config (--cpu-steps steps; an ensemble: that many "
                          "steps of each system in turn) and print its record")
@@ -298,29 +383,42 @@ def main():
     wanted = set(f for f in forms_arg.split(",") if f) | {form}
     if systems is not None:
         wanted.discard("plain")
-    for name in ("plain", "mixed", "learned"):
-        if name not in wanted:
-            continue
-        log(f"bench: {name}, warmup {warmup}, steps {steps}")
-        eng, wall, ph, extra = run_form(bh, forms[name], pos, vel, mass, theta, steps, warmup, systems=systems,
-                                        tune=name != "plain")
+    work = None   # algorithmic walk work of the headline form's final state
+    for name in ("plain", "mixed", "learned", "prev_cost"):
+        if name == "prev_cost":
+            # the schedule LEARNED replaces: heaviest first by the previous
+            # step's recorded work, on the same states (MIXED and LEARNED
+            # forces are bit-identical) with LEARNED's tuned group shape
+            if "learned" not in results or args.no_prev_cost:
+                continue
+            log(f"bench: mixed with the previous-step schedule, warmup {warmup}, steps {steps}")
+            shape = results["learned"]["autotune"]
+            eng, wall, ph, extra = run_form(bh, bh.FORM_MIXED, pos, vel, mass, theta, steps, warmup,
+                                            systems=systems, sched=bh.SCHED_PREV_COST, shape=shape, pre_steps=50)
+        else:
+            if name not in wanted:
+                continue
+            log(f"bench: {name}, warmup {warmup}, steps {steps}")
+            eng, wall, ph, extra = run_form(bh, forms[name], pos, vel, mass, theta, steps, warmup, systems=systems,
+                                            tune=name != "plain")
         r = dict(value=n * steps / wall, ms_per_step=wall / steps * 1e3,
                  phase_ms={"kick_drift_box_keys": ph[0], "sort_gather": ph[1], "tree_groups_schedule": ph[2],
                            "walk_kick": ph[3], "total_device": ph[4]},
                  phase_note=f"device phases from {PHASE_STEPS} eager steps after the timed region", **extra)
+        if name == "prev_cost":
+            r["schedule"] = "PREV_COST (heaviest first by the previous step's walk work), LEARNED's group shape"
         if systems is not None:
             r["force_error"] = ensemble_error(bh, eng, systems, 512)
         else:
             r["force_error"] = force_error(bh, eng, n, args.samples)
-        if name == "plain":
-            flops, ma, mt, mo = plain_walk_work(bh, eng, theta)
-            r["walk_work"] = dict(flops_per_launch=flops, accepted_per_body=ma, tested_per_body=mt,
-                                  opened_full_chain_per_body=mo)
-            r["walk_tflops"] = flops / (ph[3] * 1e-3) / 1e12
-        else:
+        if name != "plain":
             flops, gs = mixed_walk_work(eng)
-            r["walk_work"] = dict(flops_per_launch=flops, **gs)
-            r["walk_tflops"] = flops / (ph[3] * 1e-3) / 1e12
+            r["executed_work"] = dict(flops_per_launch=flops, **gs)
+        w = algorithmic_work_any(bh, eng, theta, systems)
+        r["algorithmic_work"] = w
+        r["walk_tflops_algorithmic"] = w["flops_per_launch"] / (ph[3] * 1e-3) / 1e12
+        if name == form:
+            work = w
         if name == "learned":
             ready, samples, updates, loss = eng.model_status()
             r["model"] = dict(ready=ready, samples=samples, updates=updates, train_loss=loss,
@@ -333,27 +431,39 @@ def main():
 
     head = results[form]
     plain = results.get("plain")
+    walk_ms = head["phase_ms"]["walk_kick"]
+    achieved = work["flops_per_launch"] / (walk_ms * 1e-3) / 1e12
     if form == "plain":
-        kname, kern, peak, bound_note = "k_walk_plain", "k_walk_plain (FP64 walk + fused half kick)", \
-            FP64_PEAK_TFLOPS, "FP64 vector"
+        kname, kern, peak, bound = "k_walk_plain", "k_walk_plain (FP64 walk + fused half kick)", \
+            FP64_PEAK_TFLOPS, "valu_fp64"
+        peak_note = "FP64 vector peak"
     else:
-        kname, kern, peak, bound_note = "k_walk_mixed", \
-            "k_walk_mixed (FP32 group walk + FP64 heavy term + fused half kick)", FP32_PEAK_TFLOPS, \
-            "FP32 vector (the published peak counts packed FP32 instructions, which the library's kernels are " \
-            "compiled without: the non-packed ceiling is 78.6)"
+        kname, kern, peak, bound = "k_walk_mixed", \
+            "k_walk_mixed (FP32 group walk + FP64 heavy term + fused half kick)", FP32_PEAK_TFLOPS, "valu_fp32"
+        peak_note = ("FP32 vector peak as published (it counts packed FP32 instructions, which the library's kernels "
+                     "are compiled without: the non-packed ceiling is 78.6)")
     tr = traffic_record(kname, args.config)
-    roofline = {"bound": "mfma", "achieved": head["walk_tflops"], "peak": peak, "unit": "TFLOP/s",
-                "frac": head["walk_tflops"] / peak, "traffic": tr["bytes_per_launch"] if tr else None,
+    roofline = {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": tr["bytes_per_launch"] if tr else None,
                 "kernel": kern,
-                "note": f"compute roofline at the {bound_note} peak; the walk runs on VALU (a gather, no dense "
-                        "contraction, so no MFMA).  Algorithmic flops per launch are counted from the walk's own "
-                        "list lengths / node tests; time is the walk phase's mean over eager steps right after "
-                        "the timed region (HIP events on the library stream; the phase is the walk kernel plus "
-                        "the small heavy-body kernels)",
-                "flops_per_launch": head["walk_work"]["flops_per_launch"], "walk_ms": head["phase_ms"]["walk_kick"]}
+                "note": f"compute roofline at the {peak_note}; the walk runs on VALU (a gather with a data-dependent "
+                        "opening decision, no dense contraction, so no MFMA).  achieved = algorithmic flops per "
+                        "launch (W = 21 I + 9 O_c per body from the plain walk's counters on the same state, "
+                        "identical for every form) / the walk phase's mean time over eager steps right after the "
+                        "timed region (HIP events on the library stream; the phase is the walk kernel plus the "
+                        "small heavy-body kernels)",
+                "algorithmic_flops_per_launch": work["flops_per_launch"], "walk_ms": walk_ms,
+                "work": work}
+    if form != "plain":
+        ex = head["executed_work"]["flops_per_launch"]
+        roofline["executed_flops_per_launch"] = ex
+        roofline["executed_over_algorithmic"] = ex / work["flops_per_launch"]
+        roofline["executed_tflops"] = ex / (walk_ms * 1e-3) / 1e12
     if tr:
         roofline["traffic_source"] = tr.get("source")
-        roofline["traffic_gbs"] = tr["bytes_per_launch"] / (head["phase_ms"]["walk_kick"] * 1e-3) / 1e9
+        roofline["traffic_gbs"] = tr["bytes_per_launch"] / (walk_ms * 1e-3) / 1e9
+        roofline["compulsory_bytes"] = work["compulsory_bytes"]
+        roofline["traffic_ratio"] = tr["bytes_per_launch"] / work["compulsory_bytes"]
 
     cpu = None
     if not args.no_cpu and systems is None:
@@ -370,6 +480,8 @@ def main():
                    "dt_days": HOUR, "form": form, **({"systems": len(systems)} if systems else {})},
         "max_rel_force_err": head["force_error"]["max_all"],
         "speedup_vs_plain": head["value"] / plain["value"] if plain else None,
+        "gain_vs_prev_cost": (head["value"] / results["prev_cost"]["value"]
+                              if form == "learned" and "prev_cost" in results else None),
         "forms": results,
         "roofline": roofline,
         "cpu_baseline": cpu,
