@@ -33,7 +33,9 @@ constexpr int kMaxDepth = 21;
 struct WalkNode {
     double x, y, z;   // centre of mass after the exchange round trip
     double m;         // mass
-    double side2;     // (L / 2^d)^2 at the test depth d (leaf: depth 20)
+    double side2;     // (L / 2^d)^2 at the test depth d; a leaf: (L / 2^20)^2 if
+                      // its chain starts above depth 21, else +inf (its own
+                      // bodies never accept it)
     uint32_t skip;    // walk index of the first node after this subtree
     uint32_t meta;    // kMeta* bits
 };

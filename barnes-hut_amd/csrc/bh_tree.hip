@@ -635,7 +635,12 @@ __global__ __launch_bounds__(kNT) void k_walk_records(size_t n, TreeBuffers tb)
         nd.z = finish_com(r[3], m);
         nd.m = 0.0 + m;
     }
-    nd.side2 = tb.sys[0].side2[leaf ? kMaxDepth - 1 : test_depth];   // the plain form runs one system
+    // the plain form runs one system.  A leaf is always accepted except by
+    // its own bodies, which accept it only when its chain starts above depth
+    // 21 and the depth-20 test passes: its side^2 is (L / 2^20)^2 for such a
+    // chain and +inf otherwise, so the walk's test alone decides the self case
+    const bool chain_leaf = leaf && top <= kMaxDepth - 1;
+    nd.side2 = leaf ? (chain_leaf ? tb.sys[0].side2[kMaxDepth - 1] : INFINITY) : tb.sys[0].side2[test_depth];
     nd.skip = skip;
     uint32_t meta = test_depth | (top << kMetaTopShift);
     if (leaf) {
