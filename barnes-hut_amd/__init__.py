@@ -113,6 +113,8 @@ SIGNATURES = [
     ("bh_set_moment_cutoff", C.c_int, [_P, C.c_int]),
     ("bh_get_sort_stats", C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(_I64)]),
     ("bh_get_relative_opening", C.c_int, [_P, C.POINTER(_D)]),
+    ("bh_compute_accelerations_static", C.c_int, [_I64, _P, _P, _D, _D, _D, _P, _P]),
+    ("bh_static_last_error", C.c_char_p, []),
 ]
 
 _lib = None

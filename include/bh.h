@@ -334,6 +334,27 @@ typedef struct bh_tune_result {
 } bh_tune_result;
 int bh_autotune(bh_ctx *ctx, int form, int reps, bh_tune_result *out);
 
+/* ------------------------------------------- evaluation without allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): one plain FP64
+ * force evaluation -- the kernels of bh_compute_accelerations with
+ * BH_FORM_PLAIN, the same bits -- on n <= BH_STATIC_MAX_BODIES bodies in
+ * the caller's device buffers, with every intermediate buffer in a static
+ * device array of the library (about 40 MiB per GPU, held until the process
+ * ends).  No context: it runs on the current device, enqueued on `stream` (a
+ * hipStream_t; NULL for the null stream), and returns without waiting.
+ * pos (n bh_vec3) and mass (n doubles) are device pointers and are not
+ * modified; acc_out (n bh_vec3, device) receives the accelerations in the
+ * input order (row i = body i; the Morton sort stays inside).  No hipMalloc,
+ * hipFree or synchronisation.  Every call on a device shares that device's
+ * static array: calls that overlap on one device race.  Argument errors
+ * return BH_ERR_ARG (message in bh_static_last_error()); device-side guard
+ * bits are not reported. */
+#define BH_STATIC_MAX_BODIES 65536
+int bh_compute_accelerations_static(int64_t n, const bh_vec3 *pos, const double *mass, double theta, double G,
+                                    double soft2, bh_vec3 *acc_out, void *stream);
+const char *bh_static_last_error(void);
+
 /* ------------------------------------------------ synthetic inputs --- */
 #define BH_GEN_BELT 0   /* central mass + 8 planets + main belt */
 #define BH_GEN_KUIPER 1 /* as BELT with 30% of the asteroids in a Kuiper belt */
