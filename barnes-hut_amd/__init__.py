@@ -392,7 +392,7 @@ class Engine:
     # ------------------------------------------------ learned cost model
     MODEL_PARAMS = 1377
 
-    def model_configure(self, train_steps=50, iters_per_step=8, batch=2048, lr=3e-3):
+    def model_configure(self, train_steps=50, iters_per_step=16, batch=2048, lr=3e-3):
         self._ck(self.lib.bh_model_configure(self.h, train_steps, iters_per_step, batch, lr))
 
     def model_reset(self, seed: int):

@@ -194,6 +194,7 @@ int reserve(bh_ctx* c, size_t n)
 
 // One force evaluation with optional leading kick + drift and trailing kick.
 constexpr int kRing = 512;
+constexpr float kHistoryDrop = 0.2f;   // share of training samples drawn without their history features
 constexpr int kHeavy = bh::kMaxHeavy;
 
 int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool kick_before, bool drift,
@@ -291,11 +292,20 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
             // iters_per_step Adam updates on mini-batches of all samples so far.
             // The first walk of a body set has no previous work
             // (feature 5 is 0 for every group): its groups are no samples.
+            // The learning rate falls along a cosine from lr to lr / 100 over
+            // the planned updates (train_steps - 1 walks with samples), so the
+            // trained weights settle instead of ending on a full-size step; a
+            // share kHistoryDrop of the drawn samples lose their history
+            // features so the network learns the no-history case too.
             if (c->prev_counts) {
                 bh::launch_append_samples(c->n, c->md.feat, tb, c->mb, c->md, s);
-                for (int k = 0; k < c->iters_per_step; ++k)
-                    bh::launch_model_train(c->md, nullptr, nullptr, 0, c->batch, 1,
-                                           static_cast<uint32_t>(c->updates++), c->lr, true, nullptr, nullptr, s);
+                const double planned = std::max(1.0, static_cast<double>(c->train_steps - 1) * c->iters_per_step);
+                for (int k = 0; k < c->iters_per_step; ++k) {
+                    const double frac = std::min(1.0, static_cast<double>(c->updates) / planned);
+                    const float lr = static_cast<float>(c->lr * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(M_PI * frac))));
+                    bh::launch_model_train(c->md, nullptr, nullptr, 0, c->batch, 1, static_cast<uint32_t>(c->updates++),
+                                           lr, true, nullptr, nullptr, s, kHistoryDrop);
+                }
             }
             if (++c->trained_steps >= c->train_steps) c->model_ready = true;
         }

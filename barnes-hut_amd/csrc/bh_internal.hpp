@@ -393,9 +393,11 @@ void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, con
 void launch_model_forward(const float* feat, size_t m, const ModelBuffers& md, float* out, hipStream_t s);
 void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, const MixedBuffers& mb,
                            const ModelBuffers& md, hipStream_t s);
+// hist_drop: share of the drawn samples whose history features (5, 6) are
+// set to zero (the in-loop training of the LEARNED form; 0 elsewhere).
 void launch_model_train(const ModelBuffers& md, const float* x, const float* t, uint64_t m, int batch,
                         int sample_mode, uint32_t iter, float lr, bool update, float* grad_out, float* loss_out,
-                        hipStream_t s);
+                        hipStream_t s, float hist_drop = 0.f);
 void launch_refresh_bf16(const ModelBuffers& md, hipStream_t s);
 
 // FP64 direct sum for m sample bodies over the bodies of each sample's system

@@ -6,12 +6,19 @@
 // layers, weights and biases stored in BF16 (round to nearest even from FP32
 // master copies), every product accumulated in FP32.  Output y is the scaled
 // log cost log2(work + 1) / 16, formed as feature 5 (the same quantity
-// from the previous step) plus the network's output, so an untrained network
-// starts from the previous-step estimate and learns corrections to it.  Loss:
-// mean squared error on y.
+// from the previous step) plus the network's output where that estimate is
+// ambiguous or missing: the output is added when feature 5 != feature 6
+// (the group's bodies come from groups of different work) or feature 5 == 0
+// (no history, e.g. the first walk after new bodies), and otherwise y is
+// feature 5 itself.  So an untrained network starts from the previous-step
+// estimate, and the network only corrects what that estimate cannot know.
+// Loss: mean squared error on y.
 // Training: Adam on the FP32 masters, BF16 weights refreshed after each
 // update; mini-batches drawn from a device ring buffer of (features, target)
-// samples filled after every walk of the training phase.  One update is two
+// samples filled after every walk of the training phase, with the history
+// features (5 and 6) of a fixed share of the drawn samples set to zero so the
+// network also learns the no-history case; the caller lowers the learning
+// rate along a cosine over the training updates.  One update is two
 // launches: per-block partial gradients over 64 samples each, then their sum
 // in block order and the Adam step.
 //
@@ -56,6 +63,11 @@ struct Act {
     float z1[kH], z2[kH];
 };
 
+// The network's output is used where the previous-step estimate (feature 5,
+// the largest previous work of the group's bodies) is ambiguous -- feature 6,
+// the smallest, differs -- or missing (zero).
+__device__ inline bool history_gate(const float* x) { return x[5] != x[6] || x[5] == 0.f; }
+
 __device__ inline float forward(const float* __restrict__ P, const float* x, Act& a)
 {
     for (int j = 0; j < kH; ++j) {
@@ -69,11 +81,10 @@ __device__ inline float forward(const float* __restrict__ P, const float* x, Act
         for (int k = 0; k < kH; ++k) s = fmaf(P[kOffW2 + j * kH + k], relu(a.z1[k]), s);
         a.z2[j] = s;
     }
-    // residual on feature 5 (the previous step's scaled walk work): the
-    // network learns the change from the fallback estimate
+    // gated residual on feature 5 (the previous step's scaled walk work)
     float y = P[kOffB3];
     for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(a.z2[j]), y);
-    return x[5] + y;
+    return history_gate(x) ? x[5] + y : x[5];
 }
 
 // ------------------------------------------------------------ features
@@ -86,68 +97,58 @@ __device__ inline T wave_reduce(T v, Op op)
     return v;
 }
 
-// The parameters for forward_wave: as load_params, but W1 and W2 stored
-// transposed ([input][unit]), so the lanes of a layer read consecutive words
-// (in the stored [unit][input] order 32 lanes read one LDS bank).
-__device__ inline void load_params_by_unit(const uint16_t* __restrict__ wb, float* s)
-{
-    for (int k = threadIdx.x; k < kModelParams; k += blockDim.x) {
-        int d = k;
-        if (k < kOffB1) d = kOffW1 + (k - kOffW1) % kF * kH + (k - kOffW1) / kF;
-        else if (k >= kOffW2 && k < kOffB2) d = kOffW2 + (k - kOffW2) % kH * kH + (k - kOffW2) / kH;
-        s[d] = from_bf16(wb[k]);
-    }
-    __syncthreads();
-}
-
 __device__ inline float lane_f32(float v, int k)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
 // forward() spread over a wavefront for one group (x: the same features in
-// every lane; P from load_params_by_unit): lane j < 32 computes hidden unit j
-// of each layer, the output runs over the units in order -- every sum in
-// forward()'s order, so the same bits.
-__device__ inline float forward_wave(const float* __restrict__ P, const float* x, int lane)
+// every lane): lane j < 32 computes hidden unit j of each layer, the output
+// runs over the units in order -- every sum in forward()'s order, so the same
+// bits.  The BF16 weights are read from global memory (2.7 KB, they stay in
+// the caches): the kernel evaluates the model for few of its groups, so it
+// stages no weights in LDS.
+__device__ inline float forward_wave_global(const uint16_t* __restrict__ wb, const float* x, int lane)
 {
     float z1 = 0.f, z2 = 0.f;
     if (lane < kH) {
-        float s = P[kOffB1 + lane];
+        float s = from_bf16(wb[kOffB1 + lane]);
 #pragma unroll
-        for (int f = 0; f < kF; ++f) s = fmaf(P[kOffW1 + f * kH + lane], x[f], s);
+        for (int f = 0; f < kF; ++f) s = fmaf(from_bf16(wb[kOffW1 + lane * kF + f]), x[f], s);
         z1 = s;
     }
     {
-        float s = lane < kH ? P[kOffB2 + lane] : 0.f;
+        float s = lane < kH ? from_bf16(wb[kOffB2 + lane]) : 0.f;
         for (int k = 0; k < kH; ++k) {
             const float a = relu(lane_f32(z1, k));
-            if (lane < kH) s = fmaf(P[kOffW2 + k * kH + lane], a, s);
+            if (lane < kH) s = fmaf(from_bf16(wb[kOffW2 + lane * kH + k]), a, s);
         }
         z2 = s;
     }
-    float y = P[kOffB3];
-    for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(lane_f32(z2, j)), y);
-    return x[5] + y;
+    float y = from_bf16(wb[kOffB3]);
+    for (int j = 0; j < kH; ++j) y = fmaf(from_bf16(wb[kOffW3 + j]), relu(lane_f32(z2, j)), y);
+    return history_gate(x) ? x[5] + y : x[5];
 }
 
 // One wavefront per group (grid-stride): lanes hold the group's bodies.  With
 // the model's weights (wb), the wavefront also evaluates the model on the
 // group's features (gcost = 2^(16 y) - 1, the predicted walk work), counts
 // the group in its block's bucket histogram, and the last block scans the
-// buckets for the heaviest-first order (sched_hist_add).
+// buckets for the heaviest-first order (sched_hist_add).  A group whose
+// history is unambiguous (features 5 and 6 equal and nonzero: about 9 in 10
+// groups) takes feature 5 as its prediction, as the model does, and forms no
+// other feature.
 __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, TreeBuffers tb, MixedBuffers mb,
                                                         float* __restrict__ feat, uint32_t max_groups,
                                                         const uint16_t* __restrict__ wb)
 {
-    __shared__ float P[kModelParams];
     __shared__ uint32_t hist[kSchedBuckets];
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;   // the walk will not use an order this step
     const int lane = threadIdx.x & 63;
     if (wb) {
         if (threadIdx.x < kSchedBuckets) hist[threadIdx.x] = 0;
-        load_params_by_unit(wb, P);   // ends with a barrier
+        __syncthreads();
     }
     const uint32_t waves = gridDim.x * (kNT / 64);
     auto dmin = [](double a, double c) { return c < a ? c : a; };
@@ -159,13 +160,27 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
         const uint32_t i0 = mb.gfirst[g], cnt = mb.gfirst[g + 1] - i0;
         const bool v = static_cast<uint32_t>(lane) < cnt;
         const uint32_t i = i0 + (v ? static_cast<uint32_t>(lane) : 0u);
+        const uint32_t pc = b.pcost[i];
+        const uint32_t cmax = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0u), umax));
+        const uint32_t cmin = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0xffffffffu), umin));
+        float f5 = flog2(static_cast<float>(cmax) + 1.f) / 16.f;
+        float f6 = flog2(static_cast<float>(cmin) + 1.f) / 16.f;
+        if (wb && f5 == f6 && f5 != 0.f) {
+            // unambiguous history: the model's output is feature 5 itself
+            // (history_gate), so the other features are not formed
+            if (lane == 0) {
+                feat[kF * static_cast<size_t>(g) + 5] = f5;
+                feat[kF * static_cast<size_t>(g) + 6] = f6;
+                const float c = fmaxf(exp2f(16.f * fminf(f5, 1.5f)) - 1.f, 0.f);
+                mb.gcost[g] = c;
+                atomicAdd(&hist[cost_bucket(c)], 1u);
+            }
+            continue;
+        }
         const double x = b.px[i], y = b.py[i], z = b.pz[i];
         const double lx = wave_reduce(x, dmin), ly = wave_reduce(y, dmin), lz = wave_reduce(z, dmin);
         const double hx = wave_reduce(x, dmax), hy = wave_reduce(y, dmax), hz = wave_reduce(z, dmax);
         const double mass = wave_reduce((v && b.hrank[i] < 0) ? b.m[i] : 0.0, dsum);
-        const uint32_t pc = b.pcost[i];
-        const uint32_t cmax = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0u), umax));
-        const uint32_t cmin = static_cast<uint32_t>(wave_reduce(static_cast<int>(v ? pc : 0xffffffffu), umin));
         // every lane forms the features (wave-uniform values); lane 0 stores them
         const double ex = hx - lx, ey = hy - ly, ez = hz - lz;
         const double ext = fmax(ex, fmax(ey, ez));
@@ -177,13 +192,13 @@ __global__ __launch_bounds__(kNT) void k_group_features(size_t n, Bodies b, Tree
         f[2] = flog2(static_cast<float>(ext) + 1e-12f) / 16.f;
         f[3] = flog2(static_cast<float>(dist) + 1e-12f) / 8.f;
         f[4] = (log10f(static_cast<float>(mass) + 1.f) - 20.f) / 5.f;
-        f[5] = flog2(static_cast<float>(cmax) + 1.f) / 16.f;
-        f[6] = flog2(static_cast<float>(cmin) + 1.f) / 16.f;
+        f[5] = f5;
+        f[6] = f6;
         f[7] = flog2(static_cast<float>(ez / (ext + 1e-30)) + 1e-6f) / 8.f;
         if (lane == 0)
             for (int k = 0; k < kF; ++k) feat[kF * static_cast<size_t>(g) + k] = f[k];
         if (wb) {
-            const float y = forward_wave(P, f, lane);
+            const float y = forward_wave_global(wb, f, lane);
             if (lane == 0) {
                 const float c = fmaxf(exp2f(16.f * fminf(y, 1.5f)) - 1.f, 0.f);
                 mb.gcost[g] = c;
@@ -251,8 +266,8 @@ constexpr int kRow = kH + 1;   // padded LDS row: lanes of a wave read distinct 
 // 8 * (t / 64) .. +7 in the forward and backward passes.
 __global__ __launch_bounds__(kNT) void k_model_grad(ModelBuffers md, const float* __restrict__ xin,
                                                     const float* __restrict__ tin, uint64_t m_in, int batch,
-                                                    int sample_mode, uint32_t iter, float* __restrict__ gpart,
-                                                    float* __restrict__ lpart)
+                                                    int sample_mode, uint32_t iter, float hist_drop,
+                                                    float* __restrict__ gpart, float* __restrict__ lpart)
 {
     __shared__ float P[kModelParams];
     __shared__ float sx[kTile][kF + 1];
@@ -273,6 +288,12 @@ __global__ __launch_bounds__(kNT) void k_model_grad(ModelBuffers md, const float
     if (t < kTile) {
 #pragma unroll
         for (int f = 0; f < kF; ++f) sx[s][f] = live ? X[kF * row + f] : 0.f;
+        // history dropout: a share hist_drop of the drawn samples (by a hash
+        // of (iter, k)) lose their history features
+        if (hist_drop > 0.f) {
+            const uint32_t hsh = mix32(iter * 0x27d4eb2du + static_cast<uint32_t>(k) * 0x165667b1u + 7u);
+            if (static_cast<float>(hsh >> 8) * (1.f / 16777216.f) < hist_drop) sx[s][5] = sx[s][6] = 0.f;
+        }
     }
     __syncthreads();
     // layer 1
@@ -294,15 +315,17 @@ __global__ __launch_bounds__(kNT) void k_model_grad(ModelBuffers md, const float
         z2[s][j] = a;
     }
     __syncthreads();
-    // output, residual on feature 5, loss and dL/dy
+    // output, gated residual on feature 5, loss and dL/dy (zero where the
+    // gate is closed: y is feature 5 there, whatever the parameters)
     if (t < kTile) {
         float y = P[kOffB3];
 #pragma unroll 8
         for (int j = 0; j < kH; ++j) y = fmaf(P[kOffW3 + j], relu(z2[s][j]), y);
-        y += sx[s][5];
+        const bool g = history_gate(sx[s]);
+        y = g ? sx[s][5] + y : sx[s][5];
         const float d = live ? y - T[row] : 0.f;
         sl[s] = d * d;
-        sdy[s] = 2.f * d / static_cast<float>(batch);
+        sdy[s] = g ? 2.f * d / static_cast<float>(batch) : 0.f;
     }
     __syncthreads();
 #pragma unroll
@@ -397,9 +420,8 @@ inline unsigned nblk(size_t n) { return static_cast<unsigned>((n + kNT - 1) / kN
 void launch_group_features(size_t n, const Bodies& b, const TreeBuffers& tb, const MixedBuffers& mb, float* feat,
                            uint32_t max_groups, const uint16_t* model, hipStream_t s)
 {
-    // one wavefront per group, grid-stride: 512 blocks up to about 300k bodies
-    // (more blocks there only add weight loads: +1% on the 300k step), more
-    // beyond so the groups' load latencies overlap (-1% at 1M with 1,750)
+    // one wavefront per group, grid-stride: 512 blocks up to about 300k bodies,
+    // more beyond so the groups' load latencies overlap (-1% at 1M with 1,750)
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>(std::max<size_t>(n / 600, 512), 4096));
     k_group_features<<<blocks, kNT, 0, s>>>(n, b, tb, mb, feat, max_groups, model);
 }
@@ -419,10 +441,10 @@ void launch_append_samples(size_t n, const float* feat, const TreeBuffers& tb, c
 
 void launch_model_train(const ModelBuffers& md, const float* x, const float* t, uint64_t m, int batch,
                         int sample_mode, uint32_t iter, float lr, bool update, float* grad_out, float* loss_out,
-                        hipStream_t s)
+                        hipStream_t s, float hist_drop)
 {
     const int nb = (batch + kTile - 1) / kTile;   // batch <= kMaxTrainBatch, checked by the callers
-    k_model_grad<<<nb, kNT, 0, s>>>(md, x, t, m, batch, sample_mode, iter, md.gpart, md.lpart);
+    k_model_grad<<<nb, kNT, 0, s>>>(md, x, t, m, batch, sample_mode, iter, hist_drop, md.gpart, md.lpart);
     k_model_apply<<<(kModelParams + kNT - 1) / kNT, kNT, 0, s>>>(md, md.gpart, md.lpart, nb, batch, iter, lr,
                                                                  update ? 1 : 0, grad_out, loss_out);
 }

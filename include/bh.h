@@ -208,11 +208,16 @@ int bh_set_moment_cutoff(bh_ctx *ctx, int cutoff);
  * An MLP (8 FP32 group features -> 32 -> 32 -> 1, ReLU, BF16 weights, FP32
  * accumulation) predicts log2(work + 1) / 16 of every walk group (work =
  * list cost + visited nodes / 2, as the walk records it, bh_get_group_work) as
- * feature 5 (the previous step's value) plus its output; the
- * LEARNED form serves groups heaviest-first by its predictions.  It trains on
- * the device from the work the walk records, during the first
- * train_steps LEARNED evaluations (iters_per_step Adam updates of `batch`
- * samples each), and is used from then on.  Features per group:
+ * feature 5 (the previous step's value) plus its output where that value is
+ * ambiguous (feature 5 != feature 6: the group's bodies come from groups of
+ * different work) or missing (feature 5 == 0: no previous walk, e.g. after
+ * bh_set_bodies), and as feature 5 alone otherwise; the LEARNED form serves
+ * groups heaviest-first by its predictions.  It trains on the device from
+ * the work the walk records, during the first train_steps LEARNED
+ * evaluations (iters_per_step Adam updates of `batch` samples each, the
+ * learning rate falling along a cosine to lr / 100, a fifth of the drawn
+ * samples stripped of features 5 and 6), and is used from then on.  Features
+ * per group:
  *   [0] tree depth / 21         [1] bodies / 64
  *   [2] log2(box extent) / 16   [3] log2(distance of box centre to origin) / 8
  *   [4] (log10(light mass + 1) - 20) / 5
@@ -245,7 +250,8 @@ int bh_model_status(bh_ctx *ctx, int *ready, int64_t *samples, int64_t *updates,
 /* Features of the groups of the last LEARNED evaluation (G x 8).  After the
  * training phase they are only computed while the walk uses the predicted
  * order, i.e. with at most 16 groups per worker wave (see
- * bh_set_schedule_split). */
+ * bh_set_schedule_split), and for a group whose features 5 and 6 are equal
+ * and nonzero (the network's output is then feature 5) only those two. */
 int bh_get_group_features(bh_ctx *ctx, int64_t capacity, float *features);
 
 /* Walk groups of the last MIXED / LEARNED evaluation: up to 64

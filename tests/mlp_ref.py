@@ -2,7 +2,11 @@
 
 8 -> 32 -> 32 -> 1 MLP, ReLU hidden layers, parameters rounded to BF16 (round
 to nearest even) for every forward pass, FP32 arithmetic, output = feature 5 +
-network output (residual on the previous-step estimate), loss mean((y - t)^2).  Parameter layout: W1[32][8] b1[32] W2[32][32] b2[32] W3[32] b3.
+g * network output (a residual on the previous-step estimate, applied only
+where that estimate is ambiguous or missing: g = 1 when feature 5 != feature
+6, i.e. the group's bodies come from groups of different work, or feature
+5 == 0, no history; else g = 0 and the output is the previous work itself),
+loss mean((y - t)^2).  Parameter layout: W1[32][8] b1[32] W2[32][32] b2[32] W3[32] b3.
 The backward pass below is the closed form of that loss; tests pin it against
 central finite differences in float64."""
 import numpy as np
@@ -35,9 +39,17 @@ def forward(p, x, dtype=np.float32, round_bf16=True):
     h1 = np.maximum(z1, 0)
     z2 = h1 @ q["W2"].astype(dtype).T + q["b2"].astype(dtype)
     h2 = np.maximum(z2, 0)
-    # residual on feature 5 (previous step's scaled walk work)
-    y = h2 @ q["W3"].astype(dtype) + q["b3"].astype(dtype)[0] + x[:, 5]
+    # gated residual on feature 5 (previous step's scaled walk work)
+    g = gate(x).astype(dtype)
+    y = x[:, 5] + g * (h2 @ q["W3"].astype(dtype) + q["b3"].astype(dtype)[0])
     return y, (x, z1, h1, z2, h2)
+
+
+def gate(x):
+    """1 where the previous-step estimate is ambiguous (feature 5 != 6) or
+    missing (feature 5 == 0), else 0."""
+    x = np.asarray(x)
+    return ((x[:, 5] != x[:, 6]) | (x[:, 5] == 0)).astype(np.float64)
 
 
 def loss_and_grad(p, x, t, dtype=np.float32, round_bf16=True):
@@ -46,7 +58,7 @@ def loss_and_grad(p, x, t, dtype=np.float32, round_bf16=True):
     m = len(t)
     d = y - np.asarray(t, dtype)
     loss = float(np.mean(d * d))
-    dy = (2.0 / m) * d
+    dy = (2.0 / m) * d * gate(x).astype(dtype)
     g = {}
     g["W3"] = h2.T @ dy
     g["b3"] = np.array([dy.sum()])
