@@ -855,30 +855,49 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 if (app) list[ln + rank_below(am)] = q;
                 ln += na;
                 n_inter += na;
-                // stack pushes: one range per opened node; a leaf of more than
-                // 8 co-keyed bodies pushes one body range per 8 bodies
+                // stack pushes: one range per opened node or leaf of at most 8
+                // bodies
                 const uint64_t pm = __ballot(npush == 1u);
                 uint64_t big = __ballot(npush > 1u);
-                uint32_t need = static_cast<uint32_t>(__popcll(pm));
-                for (uint64_t bm = big; bm; bm &= bm - 1) {
-                    const int l = __builtin_ctzll(bm);
-                    need += uni(static_cast<uint32_t>(__shfl(static_cast<int>(npush), l, 64)));
-                }
+                const uint32_t need = static_cast<uint32_t>(__popcll(pm));
                 if (sp + need > static_cast<uint32_t>(kStackCap)) {
                     if (lane == 0) atomicOr(&tb.sc->error, kErrStack);
                     overflow = true;
                     break;
                 }
                 if (npush == 1u) stk[sp + rank_below(pm)] = enc(pfirst, pcount, pbody);
-                sp += static_cast<uint32_t>(__popcll(pm));
+                sp += need;
+                // a leaf of more than 8 co-keyed bodies (duplicate or nearly
+                // coincident positions) puts its bodies on the list directly,
+                // 64 at a time, so the stack does not grow with its size
                 for (; big; big &= big - 1) {
                     const int l = __builtin_ctzll(big);
                     const uint32_t f = uni(static_cast<uint32_t>(__shfl(static_cast<int>(pfirst), l, 64)));
                     const uint32_t c = uni(static_cast<uint32_t>(__shfl(static_cast<int>(pcount), l, 64)));
-                    const uint32_t ne = (c + 7u) >> 3;
-                    for (uint32_t k = static_cast<uint32_t>(lane); k < ne; k += 64u)
-                        stk[sp + k] = enc(f + 8u * k, min(c - 8u * k, 8u), true);
-                    sp += ne;
+                    for (uint32_t k0 = 0; k0 < c; k0 += 64u) {
+                        bool appb = false;
+                        float4 qb = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (k0 + static_cast<uint32_t>(lane) < c) {
+                            const uint32_t j = f + k0 + static_cast<uint32_t>(lane);
+                            const double mj = b.m[j];
+                            if (b.hrank[j] < 0 && mj != 0.0) {
+                                appb = true;
+                                qb = make_float4(static_cast<float>(b.px[j] - cx), static_cast<float>(b.py[j] - cy),
+                                                 static_cast<float>(b.pz[j] - cz), static_cast<float>(p.G * mj));
+                            }
+                        }
+                        const uint64_t bmask = __ballot(appb);
+                        const uint32_t nb = static_cast<uint32_t>(__popcll(bmask));
+                        if (ln + nb > static_cast<uint32_t>(kListCap)) {
+                            __builtin_amdgcn_wave_barrier();
+                            n_steps += flush_list(list, ln, K, sub, cnt, ox, oy, oz, r2_0, acc, rx, ry, rz);
+                            __builtin_amdgcn_wave_barrier();
+                            ln = 0;
+                        }
+                        if (appb) list[ln + rank_below(bmask)] = qb;
+                        ln += nb;
+                        n_inter += nb;
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
             }

@@ -212,3 +212,23 @@ def test_relative_opening_tightens_dense_clusters(bh, orc, engine):
     say("trojan cloud max error: extent test only", errs[0.0].max(), "with relative test", errs[1e-7].max())
     assert errs[1e-7].max() < 1e-5
     assert errs[1e-7].max() <= errs[0.0].max()
+
+
+def test_mixed_big_coincident_leaf(bh, orc, engine):
+    """A leaf of 10,000 co-located bodies (one 21-bit cell) beside 600 others:
+    its bodies go onto the interaction list directly, so the mixed walk's
+    stack does not grow with the leaf (it used to overflow beyond about 8k
+    bodies); forces finite and, at theta 0, equal to the FP64 direct sum to
+    FP32 accuracy, the co-located terms adding nothing in both."""
+    rng = np.random.default_rng(21)
+    p = np.concatenate([np.tile([[0.7, -0.3, 0.1]], (10000, 1)), rng.uniform(-1, 1, (600, 3))])
+    mm = np.concatenate([np.full(10000, 1e18), rng.uniform(1e19, 1e21, 600)])
+    v = np.zeros_like(p)
+    for theta in (0.0, 1.05):
+        say("mixed big leaf, theta", theta)
+        acc, st = mixed_force(engine, bh, p, v, mm, theta)
+        assert np.all(np.isfinite(acc))
+        if theta == 0.0:
+            ds = orc.direct(st["pos"], st["mass"], np.arange(len(mm)))
+            err = np.linalg.norm(acc - ds, axis=1) / np.linalg.norm(ds, axis=1).max()
+            assert err.max() < 1e-4, err.max()
