@@ -102,31 +102,44 @@ __device__ inline float lane_f32(float v, int k)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
+// the two BF16 values of a 32-bit word, low half first
+__device__ inline float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ inline float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+
 // forward() spread over a wavefront for one group (x: the same features in
 // every lane): lane j < 32 computes hidden unit j of each layer, the output
 // runs over the units in order -- every sum in forward()'s order, so the same
 // bits.  The BF16 weights are read from global memory (2.7 KB, they stay in
 // the caches): the kernel evaluates the model for few of its groups, so it
-// stages no weights in LDS.
+// stages no weights in LDS.  A lane loads its rows of W1 and W2 and its
+// biases and output weight with six loads issued together, so the layers
+// wait for one memory latency, not one per weight.  (wb is the start of a
+// device allocation: the rows are 16-byte aligned.)
 __device__ inline float forward_wave_global(const uint16_t* __restrict__ wb, const float* x, int lane)
 {
-    float z1 = 0.f, z2 = 0.f;
-    if (lane < kH) {
-        float s = from_bf16(wb[kOffB1 + lane]);
+    const int j = lane < kH ? lane : 0;   // lanes past kH compute an unused copy of unit 0
+    const uint4 r1 = *reinterpret_cast<const uint4*>(wb + kOffW1 + j * kF);
+    uint4 r2[kH / 8];
 #pragma unroll
-        for (int f = 0; f < kF; ++f) s = fmaf(from_bf16(wb[kOffW1 + lane * kF + f]), x[f], s);
-        z1 = s;
+    for (int q = 0; q < kH / 8; ++q) r2[q] = *reinterpret_cast<const uint4*>(wb + kOffW2 + j * kH + 8 * q);
+    const float b1 = from_bf16(wb[kOffB1 + j]), b2 = from_bf16(wb[kOffB2 + j]);
+    const float w3 = from_bf16(wb[kOffW3 + j]), b3 = from_bf16(wb[kOffB3]);
+    const uint32_t w1[4] = {r1.x, r1.y, r1.z, r1.w};
+    float s1 = b1;
+#pragma unroll
+    for (int f = 0; f < kF; ++f) s1 = fmaf((f & 1) ? bf16_hi(w1[f >> 1]) : bf16_lo(w1[f >> 1]), x[f], s1);
+    const float z1 = s1;
+    float s2 = b2;
+#pragma unroll
+    for (int k = 0; k < kH; ++k) {
+        const uint4 r = r2[k >> 3];
+        const uint32_t wd = ((k >> 1) & 3) == 0 ? r.x : ((k >> 1) & 3) == 1 ? r.y : ((k >> 1) & 3) == 2 ? r.z : r.w;
+        s2 = fmaf((k & 1) ? bf16_hi(wd) : bf16_lo(wd), relu(lane_f32(z1, k)), s2);
     }
-    {
-        float s = lane < kH ? from_bf16(wb[kOffB2 + lane]) : 0.f;
-        for (int k = 0; k < kH; ++k) {
-            const float a = relu(lane_f32(z1, k));
-            if (lane < kH) s = fmaf(from_bf16(wb[kOffW2 + lane * kH + k]), a, s);
-        }
-        z2 = s;
-    }
-    float y = from_bf16(wb[kOffB3]);
-    for (int j = 0; j < kH; ++j) y = fmaf(from_bf16(wb[kOffW3 + j]), relu(lane_f32(z2, j)), y);
+    const float z2 = s2;
+    float y = b3;
+#pragma unroll
+    for (int u = 0; u < kH; ++u) y = fmaf(lane_f32(w3, u), relu(lane_f32(z2, u)), y);
     return history_gate(x) ? x[5] + y : x[5];
 }
 

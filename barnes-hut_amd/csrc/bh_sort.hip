@@ -687,27 +687,140 @@ __device__ inline uint32_t ns_rank(const uint64_t* ks, const uint32_t* vs, uint3
 
 // ---- merge sort of the removed keys (near mode) or of every key (full mode)
 // Tiles of kMsTile keys are sorted in LDS by a bitonic network, then rounds
-// of pairwise merges double the sorted runs: an element's place in the merged
-// run is its index in its own run plus its rank (binary search) in the other.
+// of kMsWays-way merges make the sorted runs kMsWays times longer: an
+// element's place in the merged run is its index in its own run plus its
+// ranks (binary searches, run in lockstep) in the other kMsWays - 1 runs.
 // The order is (system, key, input position), a total order, so the result
 // is the unique stable order.  Count m = ctl[1]; a round whose run length
 // reaches m has nothing to do, so rounds(m) rounds write and the result ends
 // in (k1, v1) after an even number of them, in (k2, v2) after an odd one.
+// (Measured at 300k bodies, 11.6k removed keys: eight pairwise rounds,
+// three of them with work, 47 us; the 16-way rounds: two, one with work.)
 constexpr uint32_t kMsTile = 2048;
-constexpr int kMsNT = 1024;   // threads of the tile sort: one compare-exchange each per phase
+constexpr int kMsNT = 1024;   // threads of the tile sort: two elements each
+constexpr int kMsWays = 16;
+static_assert(kMsTile == 2 * kMsNT, "the tile sort holds two elements per thread");
 
 __device__ inline uint32_t merge_rounds(uint32_t m)
 {
     uint32_t r = 0;
-    for (uint64_t L = kMsTile; L < m; L <<= 1) ++r;
+    for (uint64_t L = kMsTile; L < m; L *= kMsWays) ++r;
     return r;
 }
 
 // Tile sort: input (k2, v2) in near mode, (k0, v0) in full mode; output (k1, v1).
-// A bitonic network in LDS, 66 phases.  (Measured: 256 threads with four
-// exchanges each take 80 us at 300k bodies, these 1024 threads 35 us; 8192-key
-// tiles held in registers, shuffles for the near partners, take 144 us: the
+// A bitonic network over the tile, 66 phases.  Thread t holds elements
+// 2t and 2t + 1 in registers: the phases of partner distance 1 exchange in
+// registers, distances 2..64 across lanes by shuffles, and only distances
+// 128..1024 (10 phases) through LDS with barriers.  (Measured at 300k
+// bodies: every phase through LDS, 1024 threads, 35 us; 256 threads with
+// four exchanges each 80 us; 8192-key tiles held in registers 144 us -- the
 // few tiles of a small count leave the work to a few CUs.)
+template <bool kSys>
+__device__ inline bool ms_less(uint32_t sa, uint64_t ka, uint32_t va, uint32_t sb, uint64_t kb, uint32_t vb)
+{
+    if (kSys) return ns_less(static_cast<uint8_t>(sa), ka, va, static_cast<uint8_t>(sb), kb, vb);
+    return ka < kb || (ka == kb && va < vb);
+}
+
+__device__ inline uint64_t shfl_xor_u64(uint64_t x, int mask)
+{
+    const int lo = __shfl_xor(static_cast<int>(static_cast<uint32_t>(x)), mask, 64);
+    const int hi = __shfl_xor(static_cast<int>(static_cast<uint32_t>(x >> 32)), mask, 64);
+    return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+
+template <bool kSys>
+__device__ inline void tile_sort(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
+                                 uint64_t* __restrict__ k1, uint32_t* __restrict__ v1, const uint8_t* __restrict__ sysd,
+                                 uint32_t m, size_t t0, uint64_t* sk, uint32_t* sv, uint8_t* ss)
+{
+    const uint32_t e0 = 2u * threadIdx.x;
+    uint64_t kk[2];
+    uint32_t vv[2], sy[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const size_t x = t0 + e0 + i;
+        if (x < m) {
+            kk[i] = ik[x];
+            vv[i] = iv[x];
+            sy[i] = kSys ? sysd[vv[i]] : 0u;
+        } else {   // padding above every real entry
+            kk[i] = ~0ull;
+            vv[i] = ~0u;
+            sy[i] = 0xffu;
+        }
+    }
+    for (uint32_t k = 2; k <= kMsTile; k <<= 1) {
+        const bool asc = (e0 & k) == 0;   // the same for both elements: k >= 2
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j == 1) {
+                const bool sw = asc ? ms_less<kSys>(sy[1], kk[1], vv[1], sy[0], kk[0], vv[0])
+                                    : ms_less<kSys>(sy[0], kk[0], vv[0], sy[1], kk[1], vv[1]);
+                if (sw) {
+                    const uint64_t tk = kk[0];
+                    kk[0] = kk[1];
+                    kk[1] = tk;
+                    const uint32_t tv = vv[0];
+                    vv[0] = vv[1];
+                    vv[1] = tv;
+                    const uint32_t ts = sy[0];
+                    sy[0] = sy[1];
+                    sy[1] = ts;
+                }
+                continue;
+            }
+            // element e keeps the smaller of (e, e ^ j) when e is the lower
+            // index of an ascending pair or the upper of a descending one
+            const bool keep_min = ((e0 & j) == 0) == asc;
+            uint64_t pk[2];
+            uint32_t pv[2], ps[2];
+            if (j < 128) {
+                const int mask = static_cast<int>(j >> 1);   // partner lane: lane ^ j / 2
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    pk[i] = shfl_xor_u64(kk[i], mask);
+                    pv[i] = static_cast<uint32_t>(__shfl_xor(static_cast<int>(vv[i]), mask, 64));
+                    ps[i] = kSys ? static_cast<uint32_t>(__shfl_xor(static_cast<int>(sy[i]), mask, 64)) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    sk[e0 + i] = kk[i];
+                    sv[e0 + i] = vv[i];
+                    if (kSys) ss[e0 + i] = static_cast<uint8_t>(sy[i]);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const uint32_t q = (e0 + i) ^ j;
+                    pk[i] = sk[q];
+                    pv[i] = sv[q];
+                    ps[i] = kSys ? ss[q] : 0u;
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const bool p_less = ms_less<kSys>(ps[i], pk[i], pv[i], sy[i], kk[i], vv[i]);
+                if (keep_min == p_less) {
+                    kk[i] = pk[i];
+                    vv[i] = pv[i];
+                    sy[i] = ps[i];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const size_t x = t0 + e0 + i;
+        if (x < m) {
+            k1[x] = kk[i];
+            v1[x] = vv[i];
+        }
+    }
+}
+
 __global__ __launch_bounds__(kMsNT) void k_ns_tile_sort(const uint64_t* __restrict__ k0, const uint32_t* __restrict__ v0,
                                                         const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
                                                         uint64_t* __restrict__ k1, uint32_t* __restrict__ v1,
@@ -722,76 +835,68 @@ __global__ __launch_bounds__(kMsNT) void k_ns_tile_sort(const uint64_t* __restri
     if (t0 >= m) return;
     const uint64_t* ik = nt.ctl[0] == 1u ? k2 : k0;
     const uint32_t* iv = nt.ctl[0] == 1u ? v2 : v0;
-    for (uint32_t e = threadIdx.x; e < kMsTile; e += kMsNT) {
-        const size_t i = t0 + e;
-        if (i < m) {
-            sk[e] = ik[i];
-            sv[e] = iv[i];
-            ss[e] = sysd ? sysd[sv[e]] : 0;
-        } else {   // padding above every real entry
-            sk[e] = ~0ull;
-            sv[e] = ~0u;
-            ss[e] = 0xff;
-        }
-    }
-    __syncthreads();
-    const uint32_t p = threadIdx.x;   // kMsTile / 2 == kMsNT pairs
-    for (uint32_t k = 2; k <= kMsTile; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t lj = 31u - static_cast<uint32_t>(__builtin_clz(j));   // j = 2^lj
-            const uint32_t a = ((p >> lj) << (lj + 1)) | (p & (j - 1)), b = a + j;
-            const bool up = (a & k) == 0;
-            const bool swap = up ? ns_less(ss[b], sk[b], sv[b], ss[a], sk[a], sv[a])
-                                 : ns_less(ss[a], sk[a], sv[a], ss[b], sk[b], sv[b]);
-            if (swap) {
-                const uint64_t tk = sk[a];
-                sk[a] = sk[b];
-                sk[b] = tk;
-                const uint32_t tv = sv[a];
-                sv[a] = sv[b];
-                sv[b] = tv;
-                const uint8_t ts = ss[a];
-                ss[a] = ss[b];
-                ss[b] = ts;
-            }
-            __syncthreads();
-        }
-    }
-    for (uint32_t e = threadIdx.x; e < kMsTile; e += kMsNT) {
-        const size_t i = t0 + e;
-        if (i < m) {
-            k1[i] = sk[e];
-            v1[i] = sv[e];
-        }
-    }
+    if (sysd) tile_sort<true>(ik, iv, k1, v1, sysd, m, t0, sk, sv, ss);
+    else tile_sort<false>(ik, iv, k1, v1, sysd, m, t0, sk, sv, ss);
 }
 
-// element i of a merge round (runs of length L, count m)
+// Element i of a merge round (runs of length L, count m): its run r within
+// a group of kMsWays runs, and its ranks in the group's other runs by binary
+// searches in lockstep (the loads of one step issued together).  A lower-
+// bound search over len entries ends within floor(log2(len)) + 1 steps, so
+// 32 - clz(L) steps finish every run of the round.
 __device__ inline void merge_one(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
                                  uint64_t* __restrict__ ok, uint32_t* __restrict__ ov, const uint8_t* __restrict__ sysd,
                                  uint32_t m, uint32_t L, size_t i)
 {
-    const size_t start = i / (2 * static_cast<size_t>(L)) * (2 * static_cast<size_t>(L));
-    const size_t a1 = min(start + L, static_cast<size_t>(m)), b1 = min(start + 2 * static_cast<size_t>(L), static_cast<size_t>(m));
+    const size_t span = static_cast<size_t>(L) * kMsWays;
+    const size_t gstart = i / span * span;
+    const uint32_t r = static_cast<uint32_t>((i - gstart) / L);
     const uint64_t k = ik[i];
     const uint32_t v = iv[i];
     const uint8_t s = sysd ? sysd[v] : 0;
-    size_t pos;
-    if (i < a1) {
-        if (a1 >= b1) {   // no partner run
-            ok[i] = k;
-            ov[i] = v;
-            return;
-        }
-        pos = i + ns_rank(ik + a1, iv + a1, static_cast<uint32_t>(b1 - a1), sysd, s, k, v);
-    } else {
-        pos = start + (i - a1) + ns_rank(ik + start, iv + start, static_cast<uint32_t>(a1 - start), sysd, s, k, v);
+    size_t lo[kMsWays];
+    uint32_t len[kMsWays];
+#pragma unroll
+    for (int q = 0; q < kMsWays; ++q) {
+        const size_t a = gstart + static_cast<size_t>(q) * L;
+        const size_t e = min(a + L, static_cast<size_t>(m));
+        lo[q] = a;
+        len[q] = (static_cast<uint32_t>(q) == r || a >= e) ? 0u : static_cast<uint32_t>(e - a);
     }
+    const int steps = 32 - __builtin_clz(L);
+    for (int it = 0; it < steps; ++it) {
+        size_t mid[kMsWays];
+        uint64_t km[kMsWays];
+        uint32_t vm[kMsWays];
+#pragma unroll
+        for (int q = 0; q < kMsWays; ++q) {
+            mid[q] = lo[q] + (len[q] >> 1);
+            const size_t at = len[q] ? mid[q] : i;   // a finished search reads the element itself
+            km[q] = ik[at];
+            vm[q] = iv[at];
+        }
+#pragma unroll
+        for (int q = 0; q < kMsWays; ++q) {
+            if (len[q] == 0) continue;
+            const uint32_t half = len[q] >> 1;
+            if (ns_less(sysd ? sysd[vm[q]] : 0, km[q], vm[q], s, k, v)) {
+                lo[q] = mid[q] + 1;
+                len[q] -= half + 1;
+            } else {
+                len[q] = half;
+            }
+        }
+    }
+    size_t pos = i;   // gstart + (i - start of run r) + ranks in the other runs
+#pragma unroll
+    for (int q = 0; q < kMsWays; ++q)
+        if (static_cast<uint32_t>(q) != r) pos += lo[q] - (gstart + static_cast<size_t>(q) * L);
+    pos -= static_cast<size_t>(r) * L;
     ok[pos] = k;
     ov[pos] = v;
 }
 
-// One round: runs of length L in (ik, iv) merged pairwise into (ok, ov).
+// One round: runs of length L in (ik, iv) merged kMsWays at a time into (ok, ov).
 __global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
                                                         uint64_t* __restrict__ ok, uint32_t* __restrict__ ov,
                                                         const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap,
@@ -887,7 +992,7 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     uint32_t* va = v1;
     uint64_t* kb = k2;
     uint32_t* vb = v2;
-    for (size_t L = kMsTile; L < n; L <<= 1) {
+    for (size_t L = kMsTile; L < n; L *= kMsWays) {
         k_ns_merge_round<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(ka, va, kb, vb, sysd, tmp, cap,
                                                                                   static_cast<uint32_t>(L));
         std::swap(ka, kb);

@@ -474,56 +474,71 @@ __global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
 // Moments of branching node bi from its existing children, summed in octant
 // order 0..7 from 0.0 (src/linear_octree.cpp:69-87): the raw moments for the
 // plain form, or, when the mixed forms set tb.lraw, the light moments and
-// light-body boxes (the mixed walk never reads the raw moments).
+// light-body boxes (the mixed walk never reads the raw moments).  Every
+// child's values are loaded before the first sum (an empty octant loads the
+// node's first child again and is then skipped), so one node costs two
+// dependent memory round trips, not one per child.
 __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi)
 {
-    const int32_t* c = tb.child + 8 * bi;
+    const int4* c4 = reinterpret_cast<const int4*>(tb.child + 8 * bi);
+    const int4 ca = c4[0], cb = c4[1];
+    const int32_t c[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    int32_t any = c[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) any = (any < 0) ? c[k] : any;
+    size_t ix[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ix[k] = static_cast<size_t>(c[k] < 0 ? any : c[k]);
     if (!tb.lraw) {
+        double4 r[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r[k] = reinterpret_cast<const double4*>(tb.raw)[ix[k]];
         double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int32_t ck = c[k];
-            if (ck < 0) continue;
-            const double* r = tb.raw + 4 * static_cast<size_t>(ck);
-            sm += r[0];
-            sx += r[1];
-            sy += r[2];
-            sz += r[3];
+            if (c[k] < 0) continue;
+            sm += r[k].x;
+            sx += r[k].y;
+            sy += r[k].z;
+            sz += r[k].w;
         }
-        double* r = tb.raw + 4 * (static_cast<size_t>(M) + bi);
-        r[0] = sm;
-        r[1] = sx;
-        r[2] = sy;
-        r[3] = sz;
+        reinterpret_cast<double4*>(tb.raw)[static_cast<size_t>(M) + bi] = make_double4(sm, sx, sy, sz);
     } else {
-        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, ls = 0.0;
-        double bl[3] = {INFINITY, INFINITY, INFINITY}, bu[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double4 q[8];
+        double sec[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int32_t ck = c[k];
-            if (ck < 0) continue;
-            const double* q = tb.lraw + 4 * static_cast<size_t>(ck);
-            lm += q[0];
-            lx += q[1];
-            ly += q[2];
-            lz += q[3];
-            ls += tb.lsec[ck];
-            const double* cb = tb.lbox + 6 * static_cast<size_t>(ck);
-            for (int a = 0; a < 3; ++a) {
-                bl[a] = dmin(bl[a], cb[a]);
-                bu[a] = dmax(bu[a], cb[3 + a]);
+            q[k] = reinterpret_cast<const double4*>(tb.lraw)[ix[k]];
+            sec[k] = tb.lsec[ix[k]];
+        }
+        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, ls = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (c[k] < 0) continue;
+            lm += q[k].x;
+            lx += q[k].y;
+            ly += q[k].z;
+            lz += q[k].w;
+            ls += sec[k];
+        }
+        double bx[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+            double cbv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cbv[k] = tb.lbox[6 * ix[k] + a];
+            double v = (a < 3) ? INFINITY : -INFINITY;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (c[k] < 0) continue;
+                v = (a < 3) ? dmin(v, cbv[k]) : dmax(v, cbv[k]);
             }
+            bx[a] = v;
         }
-        double* bx = tb.lbox + 6 * (static_cast<size_t>(M) + bi);
-        for (int a = 0; a < 3; ++a) {
-            bx[a] = bl[a];
-            bx[3 + a] = bu[a];
-        }
-        double* q = tb.lraw + 4 * (static_cast<size_t>(M) + bi);
-        q[0] = lm;
-        q[1] = lx;
-        q[2] = ly;
-        q[3] = lz;
+        double* ob = tb.lbox + 6 * (static_cast<size_t>(M) + bi);
+#pragma unroll
+        for (int a = 0; a < 6; ++a) ob[a] = bx[a];
+        reinterpret_cast<double4*>(tb.lraw)[static_cast<size_t>(M) + bi] = make_double4(lm, lx, ly, lz);
         tb.lsec[static_cast<size_t>(M) + bi] = ls;
     }
 }
@@ -538,29 +553,42 @@ __global__ __launch_bounds__(kNT) void k_moments_level(TreeBuffers tb, int depth
     node_moments(tb, M, bi);
 }
 
-// The listed deep nodes (depth >= deep_from), bottom-up by depth in one
-// workgroup: a barrier between depths makes a level's moments visible to the
-// next, so each node sees final children, as with one launch per depth.  The
-// list is in LDS when it fits, else read from memory on every level.
+// The listed nodes tb.deep[0, n_deep), bottom-up by depth in one workgroup:
+// a barrier between depths makes a level's moments visible to the next, so
+// each node sees final children, as with one launch per depth.  The list is
+// in LDS when it fits, else read from memory on every level; only the
+// depths between the list's shallowest and deepest are visited.
 constexpr int kDeepNT = 1024;
 constexpr uint32_t kDeepLds = 4096;
 
-__global__ __launch_bounds__(kDeepNT) void k_moments_deep(TreeBuffers tb)
+__device__ inline void moments_listed(const TreeBuffers& tb)
 {
     __shared__ uint32_t sn[kDeepLds];
     __shared__ uint8_t sd[kDeepLds];
+    __shared__ int s_top, s_bot;
     const uint32_t M = tb.sc->n_leaves, nd = tb.sc->n_deep;
     if (nd == 0) return;
     const bool lds = nd <= kDeepLds;
-    if (lds) {
-        for (uint32_t k = threadIdx.x; k < nd; k += kDeepNT) {
-            const uint32_t bi = tb.deep[k];
-            sn[k] = bi;
-            sd[k] = tb.bdepth[bi];
-        }
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        s_top = 0;
+        s_bot = kMaxDepth;
     }
-    for (int d = kMaxDepth - 1; d >= tb.deep_from; --d) {
+    __syncthreads();
+    int top = 0, bot = kMaxDepth;
+    for (uint32_t k = threadIdx.x; k < nd; k += kDeepNT) {
+        const uint32_t bi = tb.deep[k];
+        const int d = tb.bdepth[bi];
+        top = max(top, d);
+        bot = min(bot, d);
+        if (lds) {
+            sn[k] = bi;
+            sd[k] = static_cast<uint8_t>(d);
+        }
+    }
+    atomicMax(&s_top, top);
+    atomicMin(&s_bot, bot);
+    __syncthreads();
+    for (int d = s_top; d >= s_bot; --d) {
         for (uint32_t k = threadIdx.x; k < nd; k += kDeepNT) {
             const uint32_t bi = lds ? sn[k] : tb.deep[k];
             const int dk = lds ? sd[k] : tb.bdepth[bi];
@@ -569,6 +597,9 @@ __global__ __launch_bounds__(kDeepNT) void k_moments_deep(TreeBuffers tb)
         __syncthreads();
     }
 }
+
+// the deep nodes (depth >= deep_from, listed by k_branch_init)
+__global__ __launch_bounds__(kDeepNT) void k_moments_deep(TreeBuffers tb) { moments_listed(tb); }
 
 // reversed per-leaf node counts for the suffix scan
 __global__ __launch_bounds__(kNT) void k_suffix_input(size_t n, TreeBuffers tb, uint32_t* __restrict__ out)

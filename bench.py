@@ -65,6 +65,7 @@ def log(*a):
 
 
 PHASE_STEPS = 5
+MOMENT_CUT = None      # --moment-cut: the engine's tree-moment path (None: its default)
 
 
 def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, systems=None, tune=False,
@@ -76,6 +77,8 @@ def run_form(bh, form, pos, vel, mass, theta, steps, warmup, train_steps=50, sys
     (a tuned group shape) / pre_steps (untimed steps before the warmup) let a
     MIXED run time another schedule on the states a LEARNED run times."""
     eng = bh.Engine(form=form, theta=theta)
+    if MOMENT_CUT is not None:
+        eng.set_moment_cutoff(MOMENT_CUT)
     eng.set_bodies(pos, vel, mass)
     if systems is not None:
         eng.set_systems(systems)
@@ -319,7 +322,11 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB ten times (calibrates rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE against known bytes) and exit")
+    ap.add_argument("--moment-cut", type=int, default=None,
+                    help="deep cutoff of the tree moments: -1 chosen at graph capture (default), 0..21 fixed")
     args = ap.parse_args()
+    global MOMENT_CUT
+    MOMENT_CUT = args.moment_cut
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU (one GPU holds the whole problem); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)

@@ -59,9 +59,9 @@ def test_model_training_reduces_loss(bh):
 
 def test_schedule_costs_are_the_model_predictions(bh):
     """The features kernel evaluates the model once per group, spread over a
-    wavefront with the weights in LDS by unit; its cost estimates must be the
-    model's predictions (forward(), same operation order) on the same
-    features."""
+    wavefront (lane j: hidden unit j, its weights read from global memory);
+    its cost estimates must be the model's predictions (forward(), same
+    operation order) on the same features."""
     pos, vel, m = _trojan(bh, 60000, seed=4)
     with bh.Engine(form=bh.FORM_LEARNED, theta=1.05) as eng:
         eng.set_graph(False)
