@@ -215,7 +215,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     if (ev) HIP_TRY(c, hipEventRecord(ev[k], s))
     MARK(0);
     bh::launch_kick_drift_box(c->cur, c->n, kick_before, half_dt, drift, dt, c->tb, c->partials, s);
-    bh::launch_keys(c->cur, c->n, c->tb, s);
+    bh::launch_keys(c->cur, c->n, c->tb, c->partials, s);
     MARK(1);
     if (c->sort_near)
         bh::sort_near_sorted(c->tb.key, c->tb.perm, c->tb.key_alt, c->tb.perm_alt, c->tb.key_alt2, c->tb.perm_alt2,

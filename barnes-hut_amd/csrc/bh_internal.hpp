@@ -75,7 +75,6 @@ struct StepScalars {
     uint32_t n_nodes;    // M + B
     uint32_t pad0;
     uint64_t n_full;     // node count of the reference's full-chain map
-    uint32_t block_counter;  // last-block-done counter, left at 0
     uint32_t error;          // kErr* bits raised by bounds guards
     uint32_t n_groups;       // walk groups of the mixed forms
     uint32_t group_counter;  // dynamic work queue of the mixed walk (cost order, split schedule)
@@ -236,7 +235,7 @@ constexpr unsigned kBoxBodiesPerBlock = 1024;   // 4 bodies per thread
 void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt, bool drift, double dt,
                            const TreeBuffers& tb, double* partials, hipStream_t s);
 // Morton keys (src/morton_keys.cpp:65-98) in each body's system + identity payload.
-void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s);
+void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, const double* partials, hipStream_t s);
 // Gather every body array through perm into dst; first-of-key flags (a new
 // system also starts a leaf); heavy slots of every system.
 void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s);

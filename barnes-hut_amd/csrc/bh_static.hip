@@ -227,7 +227,7 @@ int bh_compute_accelerations_static(int64_t n, const bh_vec3* pos, const double*
     static_assert(sizeof(SysScalars) / sizeof(uint32_t) <= kNT, "block 0 clears all of SysScalars");
     k_static_init<<<init_blocks, kNT, 0, s>>>(N, pos, mass, L.cur, L.tb, L.sort_words, L.scan_words);
     launch_kick_drift_box(L.cur, N, false, 0.0, false, 0.0, L.tb, L.partials, s);
-    launch_keys(L.cur, N, L.tb, s);
+    launch_keys(L.cur, N, L.tb, L.partials, s);
     radix_sort_pairs(L.tb.key, L.tb.perm, L.tb.key_alt, L.tb.perm_alt, N, 63, nullptr, L.tb.sort_tmp,
                      kStaticMaxBodies, s);
     launch_permute(L.cur, L.alt, L.tb, N, s);

@@ -96,9 +96,11 @@ __device__ inline uint32_t lower_bound_in(const uint64_t* __restrict__ u, uint32
 
 // ------------------------------------------------------ kick/drift/box
 // System s owns blocks [box_block(s), box_block(s + 1)) of the grid (about
-// one per 1024 of its bodies), which stride over its bodies; the block that
-// finishes last reduces every system's partial boxes, one wavefront per
-// system.  Min and max are exact, so the box does not depend on the partition.
+// one per 1024 of its bodies), which stride over its bodies; each block
+// writes its partial box, and k_keys, the next launch, combines a system's
+// partials where it needs them (no per-block release and last-block counter:
+// those cost about 50 ns per block, serialised).  Min and max are exact, so
+// the box does not depend on the partition.
 __device__ inline uint32_t box_first(const TreeBuffers& tb, uint32_t s)
 {
     return s < tb.n_sys ? tb.sys[s].box_block : tb.box_blocks;
@@ -145,9 +147,8 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
             hi[a] = dmax(hi[a], x[a]);
         }
     }
-    // block reduction
+    // block reduction into this block's partial box (k_keys combines them)
     __shared__ double red[6][kNT / 64];
-    __shared__ bool last;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -162,76 +163,11 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
             red[3 + a][w] = hi[a];
         }
     __syncthreads();
-    StepScalars* sc = tb.sc;
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 3; ++a) {
-            double l = red[a][0], h = red[3 + a][0];
-            for (int k = 1; k < kNT / 64; ++k) {
-                l = dmin(l, red[a][k]);
-                h = dmax(h, red[3 + a][k]);
-            }
-            partials[6 * blockIdx.x + a] = l;
-            partials[6 * blockIdx.x + 3 + a] = h;
-        }
-        __threadfence();
-        const uint32_t prev = atomicAdd(&sc->block_counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // the last block: one wavefront per system reduces its per_sys partials
-    for (uint32_t q = static_cast<uint32_t>(w); q < tb.n_sys; q += kNT / 64) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = INFINITY;
-            hi[a] = -INFINITY;
-        }
-        const uint32_t q0 = box_first(tb, q), q1 = box_first(tb, q + 1);
-        for (uint32_t k = q0 + static_cast<uint32_t>(lane); k < q1; k += 64) {
-            const size_t blk = k;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = dmin(lo[a], __hip_atomic_load(&partials[6 * blk + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                hi[a] = dmax(hi[a], __hip_atomic_load(&partials[6 * blk + 3 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            for (int o = 32; o > 0; o >>= 1) {
-                lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
-                hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
-            }
-        }
-        if (lane == 0) {
-            SysScalars* sy = tb.sys + q;
-            double bb[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
-            for (int k = 0; k < 6; ++k) sy->bb[k] = bb[k];
-            // src/morton_keys.cpp:71-73
-            for (int a = 0; a < 3; ++a) sy->span[a] = (bb[3 + a] == bb[a]) ? 1.0 : (bb[3 + a] - bb[a]);
-            // src/traversal.cpp:31-43
-            const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
-            double L = ex;
-            if (L < ey) L = ey;
-            if (L < ez) L = ez;
-            for (int d = 0; d <= kMaxDepth; ++d) {
-                const double side = L / static_cast<double>(1ull << d);
-                sy->side2[d] = side * side;
-            }
-        }
-    }
-    // sc->error is not cleared here: the guard bits stay raised across the
-    // steps of one call until the host reads and clears them
-    if (threadIdx.x == 0) {
-        sc->group_counter = 0;
-        sc->block_counter = 0;
-        sc->heavy_counter = 0;
-        sc->n_deep = 0;
-    }
-    if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
-    if (threadIdx.x < kSchedBuckets) {
-        sc->sched_hist[threadIdx.x] = 0;
-        sc->sched_cursor[threadIdx.x] = 0;
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        double v = red[a][0];
+        for (int k = 1; k < kNT / 64; ++k) v = a < 3 ? dmin(v, red[a][k]) : dmax(v, red[a][k]);
+        partials[6 * blockIdx.x + a] = v;
     }
 }
 
@@ -256,14 +192,86 @@ __device__ inline uint32_t quantise(double v, double lo, double span)
     return static_cast<uint32_t>(t * 2097152.0);
 }
 
-__global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, TreeBuffers tb)
+// Keys of one block of bodies.  The block first combines the partial boxes
+// of the systems its bodies belong to (one wavefront per system, in LDS);
+// the block holding a system's first body also stores that system's box,
+// key spans and side^2 per depth for the later kernels, and block 0 clears
+// the step's work counters.
+__global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, TreeBuffers tb, const double* __restrict__ partials)
 {
-    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    __shared__ double sbb[kMaxSystems][6];
+    const size_t i0 = static_cast<size_t>(blockIdx.x) * kNT;
+    if (i0 >= n) return;   // uniform per block
+    const size_t i1 = min(i0 + static_cast<size_t>(kNT), n);
+    const uint32_t s0 = tb.bsys[i0], s1 = tb.bsys[i1 - 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t q = s0 + static_cast<uint32_t>(w); q <= s1; q += kNT / 64) {
+        double lo[3] = {INFINITY, INFINITY, INFINITY};
+        double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const uint32_t q0 = box_first(tb, q), q1 = box_first(tb, q + 1);
+        for (uint32_t k = q0 + static_cast<uint32_t>(lane); k < q1; k += 64) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = dmin(lo[a], partials[6 * static_cast<size_t>(k) + a]);
+                hi[a] = dmax(hi[a], partials[6 * static_cast<size_t>(k) + 3 + a]);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            for (int o = 32; o > 0; o >>= 1) {
+                lo[a] = dmin(lo[a], __shfl_xor(lo[a], o, 64));
+                hi[a] = dmax(hi[a], __shfl_xor(hi[a], o, 64));
+            }
+        }
+        if (lane == 0) {
+            double* bb = sbb[q - s0];
+            for (int a = 0; a < 3; ++a) {
+                bb[a] = lo[a];
+                bb[3 + a] = hi[a];
+            }
+            SysScalars* sy = tb.sys + q;
+            if (sy->body_lo >= i0 && sy->body_lo < i1) {
+                for (int k = 0; k < 6; ++k) sy->bb[k] = bb[k];
+                // src/morton_keys.cpp:71-73
+                for (int a = 0; a < 3; ++a) sy->span[a] = (bb[3 + a] == bb[a]) ? 1.0 : (bb[3 + a] - bb[a]);
+                // src/traversal.cpp:31-43
+                const double ex = bb[3] - bb[0], ey = bb[4] - bb[1], ez = bb[5] - bb[2];
+                double L = ex;
+                if (L < ey) L = ey;
+                if (L < ez) L = ez;
+                for (int d = 0; d <= kMaxDepth; ++d) {
+                    const double side = L / static_cast<double>(1ull << d);
+                    sy->side2[d] = side * side;
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0) {
+        // sc->error is not cleared here: the guard bits stay raised across the
+        // steps of one call until the host reads and clears them
+        StepScalars* sc = tb.sc;
+        if (threadIdx.x == 0) {
+            sc->group_counter = 0;
+            sc->heavy_counter = 0;
+            sc->n_deep = 0;
+        }
+        if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
+        if (threadIdx.x < kSchedBuckets) {
+            sc->sched_hist[threadIdx.x] = 0;
+            sc->sched_cursor[threadIdx.x] = 0;
+        }
+    }
+    __syncthreads();
+    const size_t i = i0 + threadIdx.x;
     if (i >= n) return;
-    const SysScalars* sy = tb.sys + tb.bsys[i];
-    const uint32_t qx = quantise(b.px[i], sy->bb[0], sy->span[0]);
-    const uint32_t qy = quantise(b.py[i], sy->bb[1], sy->span[1]);
-    const uint32_t qz = quantise(b.pz[i], sy->bb[2], sy->span[2]);
+    const double* bb = sbb[tb.bsys[i] - s0];
+    // the spans as stored above (src/morton_keys.cpp:71-73)
+    const double sx = (bb[3] == bb[0]) ? 1.0 : (bb[3] - bb[0]);
+    const double sy_ = (bb[4] == bb[1]) ? 1.0 : (bb[4] - bb[1]);
+    const double sz = (bb[5] == bb[2]) ? 1.0 : (bb[5] - bb[2]);
+    const uint32_t qx = quantise(b.px[i], bb[0], sx);
+    const uint32_t qy = quantise(b.py[i], bb[1], sy_);
+    const uint32_t qz = quantise(b.pz[i], bb[2], sz);
     tb.key[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
     tb.perm[i] = static_cast<uint32_t>(i);
 }
@@ -697,9 +705,9 @@ void launch_kick_drift_box(const Bodies& b, size_t n, bool kick, double half_dt,
     k_kick_drift_box<<<tb.box_blocks, kNT, 0, s>>>(b, kick ? 1 : 0, half_dt, drift ? 1 : 0, dt, tb, partials);
 }
 
-void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, hipStream_t s)
+void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, const double* partials, hipStream_t s)
 {
-    k_keys<<<nblk(n), kNT, 0, s>>>(b, n, tb);
+    k_keys<<<nblk(n), kNT, 0, s>>>(b, n, tb, partials);
 }
 
 void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s)
