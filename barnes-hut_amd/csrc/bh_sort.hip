@@ -467,7 +467,8 @@ __global__ __launch_bounds__(kNT) void k_sort_pass(const uint64_t* __restrict__ 
 //
 // Control words (ctl): [0] mode (1 near, 0 full), [1] count of the merge sort
 // (removed keys, or n), [2] kept keys (0 in full mode), [3] keys of the sorted
-// run the last merge takes (= [1]), [5] removed keys.  (Measured at 300k
+// run the last merge takes (= [1]), [5] removed keys, [6] kept keys and [7]
+// the tiles' checks as k_ns_tile_scan reduced them.  (Measured at 300k
 // bodies, 11.6k removed keys: the radix sort of the removed keys -- a
 // histogram and eight look-back passes of about 11 us each -- 106 us; one
 // workgroup's LDS radix sort of them 94 us; the merge sort 58 us.)
@@ -604,11 +605,55 @@ __global__ __launch_bounds__(kNT) void k_ns_mark(const uint64_t* __restrict__ ke
     }
 }
 
+// Many tiles (2048-key tiles, above 2^20 keys): one workgroup turns the tile
+// counts into their exclusive prefix in place and reduces the counts and the
+// checks into ctl[6], ctl[7], so a scatter block reads its prefix instead of
+// adding up every tile's count (T^2 reads: 100 us at 4.1M keys).
+constexpr int kNsScanNT = 1024;
+__global__ __launch_bounds__(kNsScanNT) void k_ns_tile_scan(size_t n, uint32_t* tmp, size_t cap)
+{
+    __shared__ uint32_t part[kNsScanNT];
+    __shared__ uint32_t okw[kNsScanNT / 64];
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t T = static_cast<uint32_t>(ns_tiles(n, kNsIPTLarge));
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (T + kNsScanNT - 1u) / kNsScanNT;
+    const uint32_t u0 = min(T, t * per), u1 = min(T, u0 + per);
+    uint32_t sum = 0, ok = 1;
+    for (uint32_t u = u0; u < u1; ++u) {
+        sum += nt.tcnt[u];
+        ok &= nt.tok[u];
+    }
+    part[t] = sum;
+    for (int o = 32; o > 0; o >>= 1) ok &= static_cast<uint32_t>(__shfl_xor(static_cast<int>(ok), o, 64));
+    if ((t & 63) == 0) okw[t >> 6] = ok;
+    __syncthreads();
+    for (uint32_t o = 1; o < static_cast<uint32_t>(kNsScanNT); o <<= 1) {   // inclusive scan of the run sums
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t u = u0; u < u1; ++u) {
+        const uint32_t c = nt.tcnt[u];
+        nt.tcnt[u] = run;
+        run += c;
+    }
+    if (t == 0) {
+        uint32_t all = 1;
+        for (int q = 0; q < kNsScanNT / 64; ++q) all &= okw[q];
+        nt.ctl[6] = part[kNsScanNT - 1];
+        nt.ctl[7] = all;
+    }
+}
+
 // Every block adds the kept counts of the tiles before its own and of all
-// tiles, and the tiles' checks, so all blocks reach the same mode; block 0
-// writes the control words.  Near mode: kept keys (in index order) to
-// (k1, v1) + R, removed ones to (k2, v2), both with their payloads.
-template <int IPT>
+// tiles, and the tiles' checks (or, with kPre, reads them from
+// k_ns_tile_scan), so all blocks reach the same mode; block 0 writes the
+// control words.  Near mode: kept keys (in index order) to (k1, v1) + R,
+// removed ones to (k2, v2), both with their payloads.
+template <int IPT, bool kPre>
 __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__ k0, const uint32_t* __restrict__ v0,
                                                     size_t n, uint64_t* __restrict__ k1, uint32_t* __restrict__ v1,
                                                     uint64_t* __restrict__ k2, uint32_t* __restrict__ v2,
@@ -618,7 +663,7 @@ __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__
     __shared__ uint32_t r_pre[kWaves], r_tot[kWaves], r_ok[kWaves];
     const NsTmp nt = ns_tmp_at(tmp, cap);
     constexpr int kNsTile = kNT * IPT;
-    const uint32_t T = static_cast<uint32_t>(ns_tiles(n, IPT));
+    const uint32_t T = kPre ? 0u : static_cast<uint32_t>(ns_tiles(n, IPT));
     uint32_t pre = 0, tot = 0, ok = 1;
     for (uint32_t u = threadIdx.x; u < T; u += kNT) {
         const uint32_t c = nt.tcnt[u];
@@ -644,6 +689,11 @@ __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__
         pre += r_pre[q];
         tot += r_tot[q];
         ok &= r_ok[q];
+    }
+    if (kPre) {
+        pre = nt.tcnt[blockIdx.x];
+        tot = nt.ctl[6];
+        ok = nt.ctl[7];
     }
     const uint32_t R = static_cast<uint32_t>(n) - tot;
     const bool near = ok && R <= static_cast<uint32_t>(n / 2);
@@ -1006,10 +1056,11 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
     const unsigned T = static_cast<unsigned>(ns_tiles(n));
     if (ns_ipt(n) == kNsIPTSmall) {
         k_ns_mark<kNsIPTSmall><<<T, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
-        k_ns_scatter<kNsIPTSmall><<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
+        k_ns_scatter<kNsIPTSmall, false><<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
     } else {
         k_ns_mark<kNsIPTLarge><<<T, kNT, 0, s>>>(k0, n, sysd, tmp, cap);
-        k_ns_scatter<kNsIPTLarge><<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
+        k_ns_tile_scan<<<1, kNsScanNT, 0, s>>>(n, tmp, cap);
+        k_ns_scatter<kNsIPTLarge, true><<<T, kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, tmp, cap);
     }
     (void)bits;   // the merge sort compares whole keys
     // tile sort into (k1, v1), then rounds ping-ponging k1 -> k2 -> k1 ...

@@ -59,10 +59,11 @@ def test_near_sorted_sort_matches_full_sort(bh, form_name):
     assert took_near and 0 < keys < n // 10
 
 
-def test_near_sorted_sort_many_moved_keys(bh):
+@pytest.mark.parametrize("n", [1 << 20, 3 << 19])
+def test_near_sorted_sort_many_moved_keys(bh, n):
     """A million Kuiper-belt bodies: tens of thousands of moved keys, many
-    radix tiles chained by look-back."""
-    n = 1 << 20
+    radix tiles chained by look-back.  2^20 keys take the 512-key mark tiles,
+    1.5M the 2048-key tiles with the separate tile-count scan."""
     say("near vs full sort, many moved keys", n)
     pos, vel, m = bh.generate(bh.GEN_KUIPER, 6, n)
     near, _, last = run(bh, bh.SORT_NEAR, bh.FORM_MIXED, pos, vel, m, 3, theta=0.7)
