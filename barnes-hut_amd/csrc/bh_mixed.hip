@@ -797,51 +797,34 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const uint32_t ent = ent_n;
                 const MNode nd = nd_n;
                 nd_n = fetch_node(mb, nbuf, s + 64u, total, ent_n);
-                bool app = false;
-                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-                uint32_t npush = 0, pfirst = 0, pcount = 0;
-                bool pbody = false;
-                if (act) {
+                // node entries decide without branches (the opening test runs
+                // for every lane; a wavefront mixes nodes and leaves anyway);
+                // body entries of a multi-body leaf load their body
+                const bool isbody = (ent & 0x80000000u) != 0u;
+                const uint32_t nc = nd.info & ~kMLeaf;
+                const bool leaf = (nd.info & kMLeaf) != 0u;
+                const double ex = fmax(fabs(nd.x - cx) - hx, 0.0);
+                const double ey = fmax(fabs(nd.y - cy) - hy, 0.0);
+                const double ez = fmax(fabs(nd.z - cz) - hz, 0.0);
+                const double d2 = ex * ex + ey * ey + ez * ez;
+                const bool pass = nd.size2 < theta2 * d2 &&
+                                  (nd.size2 <= kSafe2 * d2 || static_cast<double>(nd.gq) <= qlim * (d2 * d2));
+                const bool live = act && !isbody && nd.gm != 0.f;
+                bool app = live && (leaf ? nc == 1u : pass);
+                const bool push_leaf = live && leaf && nc != 1u;
+                const bool push_node = live && !leaf && !pass;
+                uint32_t npush = push_leaf ? (nc + 7u) >> 3 : (push_node ? 1u : 0u);
+                const uint32_t pfirst = nd.first, pcount = nc;
+                const bool pbody = push_leaf;
+                float4 q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
+                                       static_cast<float>(nd.z - cz), nd.gm);
+                if (act && isbody) {
                     const uint32_t id = ent & 0x7fffffffu;
-                    if (ent & 0x80000000u) {
-                        const double mb_ = b.m[id];
-                        if (b.hrank[id] < 0 && mb_ != 0.0) {
-                            app = true;
-                            q = make_float4(static_cast<float>(b.px[id] - cx), static_cast<float>(b.py[id] - cy),
-                                            static_cast<float>(b.pz[id] - cz), static_cast<float>(p.G * mb_));
-                        }
-                    } else {
-                        if (nd.gm != 0.f) {
-                            const uint32_t nc = nd.info & ~kMLeaf;
-                            if (nd.info & kMLeaf) {
-                                if (nc == 1) {   // the node holds the light body
-                                    app = true;
-                                    q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
-                                                    static_cast<float>(nd.z - cz), nd.gm);
-                                } else {
-                                    npush = (nc + 7u) >> 3;
-                                    pfirst = nd.first;
-                                    pcount = nc;
-                                    pbody = true;
-                                }
-                            } else {
-                                const double ex = fmax(fabs(nd.x - cx) - hx, 0.0);
-                                const double ey = fmax(fabs(nd.y - cy) - hy, 0.0);
-                                const double ez = fmax(fabs(nd.z - cz) - hz, 0.0);
-                                const double d2 = ex * ex + ey * ey + ez * ez;
-                                if (nd.size2 < theta2 * d2 &&
-                                    (nd.size2 <= kSafe2 * d2 || static_cast<double>(nd.gq) <= qlim * (d2 * d2))) {
-                                    app = true;
-                                    q = make_float4(static_cast<float>(nd.x - cx), static_cast<float>(nd.y - cy),
-                                                    static_cast<float>(nd.z - cz), nd.gm);
-                                } else {
-                                    npush = 1;
-                                    pfirst = nd.first;
-                                    pcount = nc;
-                                }
-                            }
-                        }
-                    }
+                    const double mb_ = b.m[id];
+                    app = b.hrank[id] < 0 && mb_ != 0.0;
+                    if (app)
+                        q = make_float4(static_cast<float>(b.px[id] - cx), static_cast<float>(b.py[id] - cy),
+                                        static_cast<float>(b.pz[id] - cz), static_cast<float>(p.G * mb_));
                 }
                 // interaction list appends (one per lane at most)
                 const uint64_t am = __ballot(app);
