@@ -448,8 +448,8 @@ int configure_systems(bh_ctx* c, const std::vector<int64_t>& counts, const doubl
     }
     if (item > c->heavy_item_cap) return fail(c, BH_ERR_STATE, "heavy work items %u > %zu", item, c->heavy_item_cap);
     // k_kick_drift_box: blocks in proportion to the bodies (one per 1024, at
-    // least one per system), scaled down to at most kBoxGrid: every block
-    // passes the last-block counter, and 4096 of them cost 0.2 ms at 4.2M
+    // least one per system), scaled down to at most kBoxGrid: k_keys reads
+    // every partial box of a system it holds bodies of
     {
         const int64_t n = off[S];
         int64_t total = 0;

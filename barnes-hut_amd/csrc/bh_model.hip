@@ -145,9 +145,9 @@ __device__ inline float forward_wave_global(const uint16_t* __restrict__ wb, con
 
 // One wavefront per group (grid-stride): lanes hold the group's bodies.  With
 // the model's weights (wb), the wavefront also evaluates the model on the
-// group's features (gcost = 2^(16 y) - 1, the predicted walk work), counts
-// the group in its block's bucket histogram, and the last block scans the
-// buckets for the heaviest-first order (sched_hist_add).  A group whose
+// group's features (gcost = 2^(16 y) - 1, the predicted walk work) and
+// counts the group in its block's bucket histogram, which each block adds to
+// the step's bucket counts (sched_hist_add) for k_sched_scatter.  A group whose
 // history is unambiguous (features 5 and 6 equal and nonzero: about 9 in 10
 // groups) takes feature 5 as its prediction, as the model does, and forms no
 // other feature.
