@@ -69,18 +69,6 @@ __device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
     return first | ((count - 1u) << 27) | (body ? 0x80000000u : 0u);   // count in [1, 8]
 }
 
-__device__ inline uint32_t unit_count(const TreeBuffers& tb, uint32_t M, uint32_t t)
-{
-    if (t < M) return tb.leaf_first[t + 1] - tb.leaf_first[t];
-    const uint32_t bi = t - M;
-    return tb.leaf_first[tb.bhi[bi] + 1] - tb.leaf_first[tb.blo[bi]];
-}
-
-__device__ inline uint32_t unit_first_body(const TreeBuffers& tb, uint32_t M, uint32_t t)
-{
-    return t < M ? tb.leaf_first[t] : tb.leaf_first[tb.blo[t - M]];
-}
-
 // Cell occupancy by depth, per system: sys[s].dcount[d] = number of
 // boundaries u[j] | u[j+1] inside system s that split at depth d, so depth d
 // holds 1 + sum_{e < d} dcount[e] non-empty cells of s.  With one system each
@@ -160,21 +148,35 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
     if (t >= static_cast<size_t>(M) + B) return;
     const uint32_t u = static_cast<uint32_t>(t);
-    const uint32_t cnt = unit_count(tb, M, u);
-    const uint32_t depth = u < M ? kMaxDepth : tb.bdepth[u - M];
-    const double emax = tb.sys[tb.bsys[unit_first_body(tb, M, u)]].emax;
-    auto good = [&](uint32_t v, uint32_t cv) {
-        return cv <= gmax && (v < M || box_extent(tb.lbox + 6 * static_cast<size_t>(v)) <= emax);
+    // The loads come in dependent rounds, each issued together (the node's
+    // leaf range, then its bodies; then, above the groups, every child's
+    // range, bodies and box) so that a node costs a few memory trips and not
+    // one per child.
+    const bool leaf = u < M;
+    const uint32_t par = tb.parent[u];
+    uint32_t lo = u, hi = u, depth = kMaxDepth;
+    if (!leaf) {
+        lo = tb.blo[u - M];
+        hi = tb.bhi[u - M];
+        depth = tb.bdepth[u - M];
+    }
+    double ub[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) ub[a] = leaf ? 0.0 : tb.lbox[6 * static_cast<size_t>(u) + a];
+    const uint32_t f = tb.leaf_first[lo];
+    const uint32_t cnt = tb.leaf_first[hi + 1] - f;
+    const double emax = tb.sys[tb.n_sys == 1 ? 0u : tb.bsys[f]].emax;
+    auto good_box = [&](bool is_leaf, uint32_t cv, const double* bx) {
+        return cv <= gmax && (is_leaf || box_extent(bx) <= emax);
     };
-    if (tb.parent[u] == kNone && good(u, cnt)) {
-        const uint32_t f = unit_first_body(tb, M, u);
+    const bool ugood = good_box(leaf, cnt, ub);
+    if (par == kNone && ugood) {
         tb.gflag[f] = 1;
         mb.gdepth[f] = static_cast<uint8_t>(depth);
         return;
     }
-    if (u < M) {
+    if (leaf) {
         if (cnt > gmax) {   // co-keyed bodies beyond one group
-            const uint32_t f = tb.leaf_first[u];
             for (uint32_t k = 0; k < cnt; k += gmax) {
                 tb.gflag[f + k] = 1;
                 mb.gdepth[f + k] = kMaxDepth;
@@ -182,32 +184,51 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
         }
         return;
     }
-    if (good(u, cnt)) return;   // inside a group formed above
-    const int32_t* c = tb.child + 8 * static_cast<size_t>(u - M);
+    if (ugood) return;   // inside a group formed above
+    const int4* c4 = reinterpret_cast<const int4*>(tb.child + 8 * static_cast<size_t>(u - M));
+    const int4 ca = c4[0], cb = c4[1];
+    const int32_t c[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    // an empty octant reads the node's first leaf and is skipped below
+    uint32_t clo[8], chi[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t cv = c[k] < 0 ? lo : static_cast<uint32_t>(c[k]);
+        clo[k] = cv < M ? cv : tb.blo[cv - M];
+        chi[k] = cv < M ? cv : tb.bhi[cv - M];
+    }
+    uint32_t cf[8], cc[8];
+    double cbx[8][6];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t cv = c[k] < 0 ? lo : static_cast<uint32_t>(c[k]);
+        cf[k] = tb.leaf_first[clo[k]];
+        cc[k] = tb.leaf_first[chi[k] + 1];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) cbx[k][a] = tb.lbox[6 * static_cast<size_t>(cv) + a];
+    }
     uint32_t cur = 0;
     bool open = false;
     double rl[3] = {0, 0, 0}, ru[3] = {0, 0, 0};   // joint light box of the open run
+#pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int32_t ck = c[k];
-        if (ck < 0) continue;
-        const uint32_t cv = static_cast<uint32_t>(ck);
-        const uint32_t cc = unit_count(tb, M, cv);
-        if (!good(cv, cc)) {
+        if (c[k] < 0) continue;
+        const uint32_t cv = static_cast<uint32_t>(c[k]);
+        const uint32_t ccount = cc[k] - cf[k];
+        const double* bx = cbx[k];
+        if (!good_box(cv < M, ccount, bx)) {
             open = false;
             continue;
         }
-        const double* bx = tb.lbox + 6 * static_cast<size_t>(cv);
         const bool empty = !(bx[0] <= bx[3]);
-        bool fits = open && cur + cc <= gmax;
+        bool fits = open && cur + ccount <= gmax;
         if (fits && !empty) {
             double e = 0.0;
             for (int a = 0; a < 3; ++a) e = fmax(e, fmax(ru[a], bx[3 + a]) - fmin(rl[a], bx[a]));
             fits = e <= emax;
         }
         if (!fits) {
-            const uint32_t f = unit_first_body(tb, M, cv);
-            tb.gflag[f] = 1;
-            mb.gdepth[f] = static_cast<uint8_t>(depth + 1);
+            tb.gflag[cf[k]] = 1;
+            mb.gdepth[cf[k]] = static_cast<uint8_t>(depth + 1);
             cur = 0;
             open = true;
             for (int a = 0; a < 3; ++a) {
@@ -215,7 +236,7 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
                 ru[a] = -INFINITY;
             }
         }
-        cur += cc;
+        cur += ccount;
         if (!empty)
             for (int a = 0; a < 3; ++a) {
                 rl[a] = fmin(rl[a], bx[a]);
