@@ -74,7 +74,7 @@ __device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
 // holds 1 + sum_{e < d} dcount[e] non-empty cells of s.  With one system each
 // wavefront counts 64 boundaries per round with one ballot per depth; with
 // several, LDS counters per (system, depth).  Blocks add their counts into
-// dcount, which k_sys_extent reads and clears.
+// dcount, which sys_extents (block 0 of k_child_count) reads and clears.
 constexpr unsigned kHistBlocks = 128;
 
 __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
@@ -115,8 +115,9 @@ __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
 // at the first depth whose non-empty cells number at least n_s / occ (with
 // occ = 32, cells there hold ~32 bodies on average).  A group much wider than
 // its neighbours' spacing would force the walk to open everything its box
-// overlaps.  Clears dcount for the next step.
-__global__ __launch_bounds__(kNT) void k_sys_extent(TreeBuffers tb, GroupParams gp)
+// overlaps.  Clears dcount for the next step.  Run by block 0 of
+// k_child_count, the launch after k_split_hist.
+__device__ inline void sys_extents(const TreeBuffers& tb, const GroupParams& gp)
 {
     for (uint32_t q = threadIdx.x; q < tb.n_sys; q += kNT) {
         SysScalars* sy = tb.sys + q;
@@ -263,10 +264,12 @@ __device__ inline uint32_t child_count(const TreeBuffers& tb, size_t bi)
     return c;
 }
 
-// block sums of the child counts (k_slots scans them)
-__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb)
+// block sums of the child counts (k_slots scans them); block 0 also sets the
+// systems' extent limits
+__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, GroupParams gp)
 {
     const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    if (blockIdx.x == 0) sys_extents(tb, gp);
     scan_block_sum(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp);
 }
 
@@ -1140,8 +1143,7 @@ void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const Mixed
 void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, hipStream_t s)
 {
     k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb);
-    k_sys_extent<<<1, kNT, 0, s>>>(tb, gp);
-    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb);
+    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, gp);
     scan_tile_offsets(tb.scan_tmp, n, nullptr, s);
     k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
 }
