@@ -99,9 +99,24 @@ __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
         if (lane == 0)
             for (int d = 0; d <= kMaxDepth; ++d) atomicAdd(&h[0][d], cnt[d]);
     } else {
-        for (uint32_t j = blockIdx.x * kNT + threadIdx.x; j + 1 < M; j += stride) {
-            const uint8_t d = tb.delta[j];
-            if (d != kNoSplit) atomicAdd(&h[tb.bsys[tb.leaf_first[j]]][d], 1u);
+        // a wavefront whose 64 boundaries all lie in one system (nearly all of
+        // them: systems hold thousands of leaves) counts them with one ballot
+        // per depth, like the single-system case; others count lane by lane
+        for (uint32_t j0 = blockIdx.x * kNT + w * 64; j0 + 1 < M; j0 += stride) {
+            const uint32_t j = j0 + static_cast<uint32_t>(lane);
+            const bool in = j + 1 < M;
+            const int d = in ? static_cast<int>(tb.delta[j]) : -1;   // kNoSplit is never a depth
+            const uint32_t sy = in ? static_cast<uint32_t>(tb.bsys[tb.leaf_first[j]]) : 0u;
+            const uint32_t s0 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(sy)));
+            if (__all(!in || sy == s0)) {
+#pragma unroll
+                for (int e = 0; e <= kMaxDepth; ++e) {
+                    const uint32_t c = static_cast<uint32_t>(__popcll(__ballot(d == e)));
+                    if (lane == 0 && c) atomicAdd(&h[s0][e], c);
+                }
+            } else if (in && d != kNoSplit) {
+                atomicAdd(&h[sy][d], 1u);
+            }
         }
     }
     __syncthreads();

@@ -216,6 +216,50 @@ __global__ __launch_bounds__(kNT) void k_scan_sums(uint32_t* __restrict__ sums, 
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// The same scan for many tiles (above one pass of k_scan_sums): 1,024
+// threads of 16 sums each, 16,384 sums a pass (one pass up to 4.2M values).
+constexpr int kWideNT = 1024, kWideIPT = 16;
+__global__ __launch_bounds__(kWideNT) void k_scan_sums_wide(uint32_t* __restrict__ sums, size_t nt,
+                                                            uint32_t* __restrict__ total_out)
+{
+    __shared__ uint32_t lds[kWideNT / 64 + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (size_t base = 0; base < nt; base += static_cast<size_t>(kWideNT) * kWideIPT) {
+        const size_t b = base + static_cast<size_t>(threadIdx.x) * kWideIPT;
+        uint32_t v[kWideIPT];
+        uint32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < kWideIPT; ++j) {
+            v[j] = (b + j < nt) ? sums[b + j] : 0u;
+            s += v[j];
+        }
+        const uint32_t inc = wave_inclusive_scan(s);
+        if (lane == 63) lds[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kWideNT / 64; ++k) {
+                const uint32_t t = lds[k];
+                lds[k] = run;
+                run += t;
+            }
+            lds[kWideNT / 64] = run;
+        }
+        __syncthreads();
+        uint32_t ex = inc - s + lds[w] + carry;
+        const uint32_t total = lds[kWideNT / 64];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kWideIPT; ++j) {
+            if (b + j < nt) sums[b + j] = ex;
+            ex += v[j];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
 __global__ __launch_bounds__(256) void k_tile_sums256(const uint32_t* __restrict__ in, size_t n,
                                                       uint32_t* __restrict__ tsum)
 {
@@ -1019,7 +1063,9 @@ size_t scan_tmp_words(size_t cap) { return std::max(scan_tiles(cap) + 1, tile_su
 
 void scan_tile_offsets(uint32_t* tsum, size_t n, uint32_t* total, hipStream_t s)
 {
-    k_scan_sums<<<1, kNT, 0, s>>>(tsum, (n + 255) / 256, total);
+    const size_t nt = (n + 255) / 256;
+    if (nt <= static_cast<size_t>(kTile)) k_scan_sums<<<1, kNT, 0, s>>>(tsum, nt, total);
+    else k_scan_sums_wide<<<1, kWideNT, 0, s>>>(tsum, nt, total);
 }
 
 void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* toff, uint32_t* out, hipStream_t s)
@@ -1038,7 +1084,8 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size
     size_t nt = (n + kTile - 1) / kTile;
     if (nt == 0) nt = 1;
     k_scan_tile_sums<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp);
-    k_scan_sums<<<1, kNT, 0, s>>>(tmp, nt, total);
+    if (nt <= static_cast<size_t>(kTile)) k_scan_sums<<<1, kNT, 0, s>>>(tmp, nt, total);
+    else k_scan_sums_wide<<<1, kWideNT, 0, s>>>(tmp, nt, total);
     k_scan_apply<<<static_cast<unsigned>(nt), kNT, 0, s>>>(in, n, tmp, out);
 }
 
