@@ -106,6 +106,79 @@ __device__ inline uint32_t box_first(const TreeBuffers& tb, uint32_t s)
     return s < tb.n_sys ? tb.sys[s].box_block : tb.box_blocks;
 }
 
+// Bodies i, i + stride, ... below i1: the half kick (src/main.cpp:196-198:
+// v += 0.5 * dt * a), the drift (src/main.cpp:205-207: x += dt * v) and the
+// running box.  Four bodies at a time, every load of the four issued before
+// the first store, so their latencies overlap (each body's arithmetic is the
+// same as one at a time; min and max are exact, so the box is too).
+template <bool kKick, bool kDrift>
+__device__ inline void kick_drift_range(const Bodies& b, size_t i, size_t i1, size_t stride, double half_dt,
+                                        double dt, double lo[3], double hi[3])
+{
+    constexpr int kU = 4;
+    auto one = [&](double (&v)[3], double (&x)[3], const double (&a)[3], size_t k) {
+        if (kKick) {
+            v[0] = v[0] + half_dt * a[0];
+            v[1] = v[1] + half_dt * a[1];
+            v[2] = v[2] + half_dt * a[2];
+            b.vx[k] = v[0];
+            b.vy[k] = v[1];
+            b.vz[k] = v[2];
+        }
+        if (kDrift) {
+            x[0] = x[0] + dt * v[0];
+            x[1] = x[1] + dt * v[1];
+            x[2] = x[2] + dt * v[2];
+            b.px[k] = x[0];
+            b.py[k] = x[1];
+            b.pz[k] = x[2];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = dmin(lo[c], x[c]);
+            hi[c] = dmax(hi[c], x[c]);
+        }
+    };
+    for (; i + (kU - 1) * stride < i1; i += kU * stride) {
+        double v[kU][3], x[kU][3], a[kU][3];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const size_t k = i + u * stride;
+            if (kKick || kDrift) {
+                v[u][0] = b.vx[k];
+                v[u][1] = b.vy[k];
+                v[u][2] = b.vz[k];
+            }
+            if (kKick) {
+                a[u][0] = b.ax[k];
+                a[u][1] = b.ay[k];
+                a[u][2] = b.az[k];
+            } else {
+                a[u][0] = a[u][1] = a[u][2] = 0.0;
+            }
+            x[u][0] = b.px[k];
+            x[u][1] = b.py[k];
+            x[u][2] = b.pz[k];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) one(v[u], x[u], a[u], i + u * stride);
+    }
+    for (; i < i1; i += stride) {
+        double v[3] = {0.0, 0.0, 0.0}, x[3] = {b.px[i], b.py[i], b.pz[i]}, a[3] = {0.0, 0.0, 0.0};
+        if (kKick || kDrift) {
+            v[0] = b.vx[i];
+            v[1] = b.vy[i];
+            v[2] = b.vz[i];
+        }
+        if (kKick) {
+            a[0] = b.ax[i];
+            a[1] = b.ay[i];
+            a[2] = b.az[i];
+        }
+        one(v, x, a, i);
+    }
+}
+
 __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, double half_dt, int drift, double dt,
                                                         TreeBuffers tb, double* __restrict__ partials)
 {
@@ -120,33 +193,11 @@ __global__ __launch_bounds__(kNT) void k_kick_drift_box(Bodies b, int kick, doub
     double lo[3] = {INFINITY, INFINITY, INFINITY};
     double hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     const size_t stride = static_cast<size_t>(per_sys) * kNT;
-    for (size_t i = i0 + static_cast<size_t>(part) * kNT + threadIdx.x; i < i1; i += stride) {
-        double v[3] = {b.vx[i], b.vy[i], b.vz[i]};
-        if (kick) {
-            // src/main.cpp:196-198: v += 0.5 * dt * a
-            v[0] = v[0] + half_dt * b.ax[i];
-            v[1] = v[1] + half_dt * b.ay[i];
-            v[2] = v[2] + half_dt * b.az[i];
-            b.vx[i] = v[0];
-            b.vy[i] = v[1];
-            b.vz[i] = v[2];
-        }
-        double x[3] = {b.px[i], b.py[i], b.pz[i]};
-        if (drift) {
-            // src/main.cpp:205-207: x += dt * v
-            x[0] = x[0] + dt * v[0];
-            x[1] = x[1] + dt * v[1];
-            x[2] = x[2] + dt * v[2];
-            b.px[i] = x[0];
-            b.py[i] = x[1];
-            b.pz[i] = x[2];
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = dmin(lo[a], x[a]);
-            hi[a] = dmax(hi[a], x[a]);
-        }
-    }
+    const size_t i = i0 + static_cast<size_t>(part) * kNT + threadIdx.x;
+    if (kick && drift) kick_drift_range<true, true>(b, i, i1, stride, half_dt, dt, lo, hi);
+    else if (kick) kick_drift_range<true, false>(b, i, i1, stride, half_dt, dt, lo, hi);
+    else if (drift) kick_drift_range<false, true>(b, i, i1, stride, half_dt, dt, lo, hi);
+    else kick_drift_range<false, false>(b, i, i1, stride, half_dt, dt, lo, hi);
     // block reduction into this block's partial box (k_keys combines them)
     __shared__ double red[6][kNT / 64];
 #pragma unroll
