@@ -749,7 +749,22 @@ __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__
         nt.ctl[5] = R;
     }
     if (!near) return;
-    const size_t base = static_cast<size_t>(blockIdx.x) * kNsTile + static_cast<size_t>(threadIdx.x) * IPT;
+    // The tile's keys and payloads come in and go out coalesced through LDS:
+    // thread t owns elements IPT t .. IPT t + IPT - 1 of the tile for the scan,
+    // and writes each one's destination (kept: R + its rank among the kept
+    // keys; removed: its rank among the removed ones) next to it in LDS.
+    __shared__ uint64_t sk[kNsTile];
+    __shared__ uint32_t sv[kNsTile], sd[kNsTile];
+    __shared__ uint8_t sf[kNsTile];
+    const size_t t0 = static_cast<size_t>(blockIdx.x) * kNsTile;
+    for (int e = threadIdx.x; e < kNsTile; e += kNT) {
+        if (t0 + e < n) {
+            sk[e] = k0[t0 + e];
+            sv[e] = v0[t0 + e];
+        }
+    }
+    const int e0 = threadIdx.x * IPT;
+    const size_t base = t0 + static_cast<size_t>(e0);
     uint32_t f[IPT], cnt = 0;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -757,19 +772,29 @@ __global__ __launch_bounds__(kNT) void k_ns_scatter(const uint64_t* __restrict__
         cnt += f[j];
     }
     uint32_t total;
-    uint32_t kept_before = pre + block_exclusive_scan(cnt, &total, lds);
+    uint32_t kept_before = pre + block_exclusive_scan(cnt, &total, lds);   // its barriers also order the LDS fill
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const size_t i = base + j;
         if (i >= n) break;
+        sf[e0 + j] = static_cast<uint8_t>(f[j]);
         if (f[j]) {
-            k1[R + kept_before] = k0[i];
-            v1[R + kept_before] = v0[i];
+            sd[e0 + j] = R + kept_before;
             ++kept_before;
         } else {
-            const size_t r = i - kept_before;
-            k2[r] = k0[i];
-            v2[r] = v0[i];
+            sd[e0 + j] = static_cast<uint32_t>(i - kept_before);
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kNsTile; e += kNT) {
+        if (t0 + e >= n) break;
+        const uint32_t d = sd[e];
+        if (sf[e]) {
+            k1[d] = sk[e];
+            v1[d] = sv[e];
+        } else {
+            k2[d] = sk[e];
+            v2[d] = sv[e];
         }
     }
 }
