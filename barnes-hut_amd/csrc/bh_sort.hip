@@ -1082,6 +1082,99 @@ __global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uin
     v0[pos] = v;
 }
 
+// The same merge by merge path, for many keys: block b writes outputs
+// [2048 b, 2048 b + 2048), whose sources are A[a0, a1) and B[o - a ...) with
+// a0, a1 found by one binary search along each end's diagonal; the two
+// segments are staged in LDS, each thread merges 8 consecutive outputs after
+// a diagonal search of its own in LDS, and the tile goes out coalesced.  The
+// order is the total order (system, key, input position), so the result is
+// the per-element merge's, with one global search per block instead of one
+// per key.
+constexpr int kMpIPT = 8;
+constexpr int kMpTile = kNT * kMpIPT;
+
+struct MpRun {
+    const uint64_t* k;
+    const uint32_t* v;
+};
+
+__device__ inline bool mp_less(const MpRun& x, size_t i, const MpRun& y, size_t j, const uint8_t* sysd)
+{
+    const uint32_t vx = x.v[i], vy = y.v[j];
+    return ns_less(sysd ? sysd[vx] : 0, x.k[i], vx, sysd ? sysd[vy] : 0, y.k[j], vy);
+}
+
+// number of A entries among the first o outputs of merge(A[0, K), B[0, R))
+__device__ inline uint32_t mp_diagonal(const MpRun& A, uint32_t K, const MpRun& B, uint32_t R, size_t o,
+                                       const uint8_t* sysd)
+{
+    uint32_t lo = o > R ? static_cast<uint32_t>(o - R) : 0u;
+    uint32_t hi = o < K ? static_cast<uint32_t>(o) : K;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (mp_less(A, mid, B, o - 1 - mid, sysd)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kNT) void k_ns_merge_path(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
+                                                       const uint64_t* __restrict__ k1, const uint32_t* __restrict__ v1,
+                                                       const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
+                                                       const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
+{
+    __shared__ uint64_t sk[kMpTile], ok[kMpTile];
+    __shared__ uint32_t sv[kMpTile], ov[kMpTile];
+    __shared__ uint8_t ss[kMpTile];
+    __shared__ uint32_t s_a[2];
+    const size_t o0 = static_cast<size_t>(blockIdx.x) * kMpTile;
+    if (o0 >= n) return;
+    const size_t o1 = min(o0 + static_cast<size_t>(kMpTile), n);
+    const NsTmp nt = ns_tmp_at(tmp, cap);
+    const uint32_t R = nt.ctl[3], K = nt.ctl[2];
+    const bool odd = merge_rounds(nt.ctl[1]) & 1u;
+    const MpRun A{k1 + R, v1 + R};
+    const MpRun B{odd ? k2 : k1, odd ? v2 : v1};
+    if (threadIdx.x < 2) s_a[threadIdx.x] = mp_diagonal(A, K, B, R, threadIdx.x == 0 ? o0 : o1, sysd);
+    __syncthreads();
+    const uint32_t a0 = s_a[0], na = s_a[1] - a0;
+    const size_t b0 = o0 - a0;
+    const uint32_t len = static_cast<uint32_t>(o1 - o0), nb = len - na;
+    for (uint32_t e = threadIdx.x; e < len; e += kNT) {
+        const bool fa = e < na;
+        const uint64_t k = fa ? A.k[a0 + e] : B.k[b0 + (e - na)];
+        const uint32_t v = fa ? A.v[a0 + e] : B.v[b0 + (e - na)];
+        sk[e] = k;
+        sv[e] = v;
+        ss[e] = sysd ? sysd[v] : 0;
+    }
+    __syncthreads();
+    // this thread's outputs q0 .. q0 + 7 of the tile: A = LDS [0, na), B = [na, len)
+    const uint32_t q0 = threadIdx.x * kMpIPT;
+    if (q0 < len) {
+        uint32_t lo = q0 > nb ? q0 - nb : 0u, hi = q0 < na ? q0 : na;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1, j = na + (q0 - 1 - mid);
+            if (ns_less(ss[mid], sk[mid], sv[mid], ss[j], sk[j], sv[j])) lo = mid + 1;
+            else hi = mid;
+        }
+        uint32_t ia = lo, ib = na + (q0 - lo);
+        const uint32_t q1 = min(q0 + kMpIPT, len);
+        for (uint32_t q = q0; q < q1; ++q) {
+            const bool take_a =
+                ib >= len || (ia < na && ns_less(ss[ia], sk[ia], sv[ia], ss[ib], sk[ib], sv[ib]));
+            const uint32_t src = take_a ? ia++ : ib++;
+            ok[q] = sk[src];
+            ov[q] = sv[src];
+        }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < len; e += kNT) {
+        k0[o0 + e] = ok[e];
+        v0[o0 + e] = ov[e];
+    }
+}
+
 }  // namespace
 
 size_t scan_tmp_words(size_t cap) { return std::max(scan_tiles(cap) + 1, tile_sum_words(cap)); }
@@ -1148,7 +1241,11 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, sysd, tmp, cap);
+    if (n > kNsSmallMax)
+        k_ns_merge_path<<<static_cast<unsigned>((n + kMpTile - 1) / kMpTile), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2,
+                                                                                          sysd, tmp, cap);
+    else
+        k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, sysd, tmp, cap);
 }
 
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
