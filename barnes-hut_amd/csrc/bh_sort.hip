@@ -807,25 +807,6 @@ __device__ inline bool ns_less(uint8_t sa, uint64_t ka, uint32_t va, uint8_t sb,
     return va < vb;
 }
 
-// number of entries of the sorted run (ks, vs)[0, len) ordered before (s, k, v)
-__device__ inline uint32_t ns_rank(const uint64_t* ks, const uint32_t* vs, uint32_t len, const uint8_t* sysd, uint8_t s,
-                                   uint64_t k, uint32_t v)
-{
-    uint32_t lo = 0;
-    while (len > 0) {
-        const uint32_t half = len >> 1;
-        const uint32_t m = lo + half;
-        const uint32_t vm = vs[m];
-        if (ns_less(sysd ? sysd[vm] : 0, ks[m], vm, s, k, v)) {
-            lo = m + 1;
-            len -= half + 1;
-        } else {
-            len = half;
-        }
-    }
-    return lo;
-}
-
 // ---- merge sort of the removed keys (near mode) or of every key (full mode)
 // Tiles of kMsTile keys are sorted in LDS by a bitonic network, then rounds
 // of kMsWays-way merges make the sorted runs kMsWays times longer: an
@@ -1051,45 +1032,16 @@ __global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restri
 }
 
 
-// Merge of the kept run ((k1, v1) + R, K keys) and the sorted removed run
-// (R keys, in (k1, v1) or (k2, v2) by merge_rounds(R)) into (k0, v0).  Full
-// mode: K = 0, a copy.
-__global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
-                                                  const uint64_t* __restrict__ k1, const uint32_t* __restrict__ v1,
-                                                  const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
-                                                  const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
-{
-    const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    if (i >= n) return;
-    const NsTmp nt = ns_tmp_at(tmp, cap);
-    const uint32_t R = nt.ctl[3], K = nt.ctl[2];
-    const bool odd = merge_rounds(nt.ctl[1]) & 1u;
-    const uint64_t* fk = odd ? k2 : k1;
-    const uint32_t* fv = odd ? v2 : v1;
-    uint64_t k;
-    uint32_t v, pos;
-    if (i < K) {
-        k = k1[R + i];
-        v = v1[R + i];
-        pos = static_cast<uint32_t>(i) + ns_rank(fk, fv, R, sysd, sysd ? sysd[v] : 0, k, v);
-    } else {
-        const uint32_t u = static_cast<uint32_t>(i) - K;
-        k = fk[u];
-        v = fv[u];
-        pos = u + ns_rank(k1 + R, v1 + R, K, sysd, sysd ? sysd[v] : 0, k, v);
-    }
-    k0[pos] = k;
-    v0[pos] = v;
-}
-
-// The same merge by merge path, for many keys: block b writes outputs
-// [2048 b, 2048 b + 2048), whose sources are A[a0, a1) and B[o - a ...) with
-// a0, a1 found by one binary search along each end's diagonal; the two
+// Merge of the kept run A = ((k1, v1) + R, K keys) and the sorted removed run
+// B (R keys, in (k1, v1) or (k2, v2) by merge_rounds(R)) into (k0, v0); full
+// mode: K = 0, a copy.  Merge path: block b writes outputs [2048 b,
+// 2048 b + 2048), whose sources are A[a0, a1) and B[o0 - a0, o1 - a1), with a0
+// and a1 found by one binary search along each end's diagonal; the two
 // segments are staged in LDS, each thread merges 8 consecutive outputs after
 // a diagonal search of its own in LDS, and the tile goes out coalesced.  The
-// order is the total order (system, key, input position), so the result is
-// the per-element merge's, with one global search per block instead of one
-// per key.
+// order is the total order (system, key, input position): the merge is
+// unique.  (A binary search per key in the other run measured the same at
+// 300k and 40 us slower at 4.2M.)
 constexpr int kMpIPT = 8;
 constexpr int kMpTile = kNT * kMpIPT;
 
@@ -1118,7 +1070,7 @@ __device__ inline uint32_t mp_diagonal(const MpRun& A, uint32_t K, const MpRun& 
     return lo;
 }
 
-__global__ __launch_bounds__(kNT) void k_ns_merge_path(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
+__global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uint32_t* __restrict__ v0, size_t n,
                                                        const uint64_t* __restrict__ k1, const uint32_t* __restrict__ v1,
                                                        const uint64_t* __restrict__ k2, const uint32_t* __restrict__ v2,
                                                        const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap)
@@ -1241,11 +1193,8 @@ void sort_near_sorted(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, ui
         std::swap(ka, kb);
         std::swap(va, vb);
     }
-    if (n > kNsSmallMax)
-        k_ns_merge_path<<<static_cast<unsigned>((n + kMpTile - 1) / kMpTile), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2,
-                                                                                          sysd, tmp, cap);
-    else
-        k_ns_merge<<<static_cast<unsigned>((n + kNT - 1) / kNT), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, sysd, tmp, cap);
+    k_ns_merge<<<static_cast<unsigned>((n + kMpTile - 1) / kMpTile), kNT, 0, s>>>(k0, v0, n, k1, v1, k2, v2, sysd, tmp,
+                                                                                cap);
 }
 
 uint32_t* sort_error_word(uint32_t* sort_tmp) { return sort_tmp_at(sort_tmp).chain + 3; }
