@@ -75,7 +75,13 @@ __device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
 // wavefront counts 64 boundaries per round with one ballot per depth; with
 // several, LDS counters per (system, depth).  Blocks add their counts into
 // dcount, which sys_extents (block 0 of k_child_count) reads and clears.
-constexpr unsigned kHistBlocks = 128;
+// Blocks: one per 4,096 bodies, at least 128 and at most 2,048, so each
+// wavefront takes about 16 rounds of 64 boundaries (with 128 blocks at 4.1M
+// bodies each took 126 rounds of dependent loads: 261 us).
+inline unsigned hist_blocks(size_t n)
+{
+    return static_cast<unsigned>(std::min<size_t>(2048, std::max<size_t>(128, n / 4096)));
+}
 
 __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
 {
@@ -1157,7 +1163,7 @@ void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const Mixed
 
 void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, hipStream_t s)
 {
-    k_split_hist<<<kHistBlocks, kNT, 0, s>>>(tb);
+    k_split_hist<<<hist_blocks(n), kNT, 0, s>>>(tb);
     k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, gp);
     scan_tile_offsets(tb.scan_tmp, n, nullptr, s);
     k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
