@@ -174,6 +174,8 @@ int reserve(bh_ctx* c, size_t n)
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
     A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.lsec, 2 * N); A(t.walk, 2 * N);
     A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems); A(t.deep, N);
+    A(t.pyr, bh::kPyrDoubles * bh::pyr_entries(N));
+    t.pyr_cap = static_cast<uint32_t>(N);
     t.bsys = c->bsys_w;
     MixedBuffers& m = c->mb;
     A(m.node, 2 * N); A(m.slot, 2 * N); A(m.base, N); A(m.gfirst, N + 1); A(m.gdepth, N);
@@ -244,11 +246,11 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
             HIP_TRY(c, hipStreamWaitEvent(c->side, c->fork, 0));
             bh::launch_mixed_layout(c->n, tb, c->mb, c->gp, c->side);
             HIP_TRY(c, hipEventRecord(c->join, c->side));
-            bh::launch_tree_moments(c->n, tb, s);
+            bh::launch_light_moments(c->n, tb, s);
             HIP_TRY(c, hipStreamWaitEvent(s, c->join, 0));
         } else {
             bh::launch_mixed_layout(c->n, tb, c->mb, c->gp, s);
-            bh::launch_tree_moments(c->n, tb, s);
+            bh::launch_light_moments(c->n, tb, s);
         }
         bh::launch_mixed_groups(c->cur, c->n, tb, c->mb, c->gp, G, s);
         int sched = form == BH_FORM_LEARNED ? BH_SCHED_MODEL : c->sched;

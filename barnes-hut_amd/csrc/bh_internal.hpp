@@ -66,6 +66,7 @@ struct SysScalars {
     uint32_t box_block;  // first block of the system in k_kick_drift_box's grid
     uint32_t heavy_slot[kMaxHeavy];   // sorted position of each heavy body, by rank
     uint32_t dcount[kMaxDepth + 1];   // boundaries splitting at each depth (mixed forms)
+    uint32_t pyr_off[28];             // first entry of the system at each level of the light-moment pyramid
 };
 
 // Device-side scalars shared by the kernels of one force evaluation.
@@ -225,7 +226,23 @@ struct TreeBuffers {
     // k_branch_init lists them in deep (capacity N).  kMaxDepth: none.
     uint32_t* deep;
     int deep_from = kMaxDepth;
+    // Light-moment pyramid of the mixed forms: level k (1..kPyrLevels) holds,
+    // per system, the combined light moments and box of its leaves
+    // [a 2^k, (a + 1) 2^k) counted from the system's first leaf.
+    double* pyr;
+    uint32_t pyr_cap;     // leaf capacity the level regions are sized for
 };
+
+constexpr int kPyrLevels = 27;    // 2^27 leaves exceed any system
+constexpr int kPyrDoubles = 12;   // lm, lx, ly, lz, ls, (pad), box min xyz, box max xyz
+__host__ __device__ inline size_t pyr_level_cap(size_t cap, int k) { return (cap >> k) + kMaxSystems + 1; }
+__host__ __device__ inline size_t pyr_level_off(size_t cap, int k)
+{
+    size_t o = 0;
+    for (int q = 1; q < k; ++q) o += pyr_level_cap(cap, q);
+    return o;
+}
+__host__ __device__ inline size_t pyr_entries(size_t cap) { return pyr_level_off(cap, kPyrLevels + 1); }
 
 // kick (optional) + drift + per-system box; writes bb/span/side2 of every
 // system.  partials hold 6 doubles per block (at most kBoxBlocks blocks).
@@ -245,6 +262,9 @@ void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb,
 // adds the branching nodes' moments (21 per-depth launches).
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s);
 void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s);
+// the mixed forms' light moments and boxes of the branching nodes, from the
+// pyramid of their leaves (four launches at every size)
+void launch_light_moments(size_t n, TreeBuffers& tb, hipStream_t s);
 // Walk-order records of the plain walk (after launch_build_tree).
 void launch_walk_order(size_t n, TreeBuffers& tb, hipStream_t s);
 

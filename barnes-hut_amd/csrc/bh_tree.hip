@@ -660,6 +660,207 @@ __device__ inline void moments_listed(const TreeBuffers& tb)
 // the deep nodes (depth >= deep_from, listed by k_branch_init)
 __global__ __launch_bounds__(kDeepNT) void k_moments_deep(TreeBuffers tb) { moments_listed(tb); }
 
+// ---------------------------------------------- light-moment pyramid
+// The mixed forms need, per branching node, the light moments (mass, mass
+// moments, second moment) and the light bodies' box over its leaf range
+// [blo, bhi] -- the order of the FP64 sums is theirs to choose, as long as it
+// depends on the system alone (the plain form keeps the reference's order,
+// k_moments_level).  So instead of one launch per depth, a pyramid over each
+// system's leaves: level k entry a combines leaves [a 2^k, (a + 1) 2^k)
+// (counted from the system's first leaf) as the pairwise sum of level k - 1
+// entries 2a and 2a + 1.  Three passes build nine levels each in LDS, and a
+// node sums the aligned pieces of its range from left to right (at most two
+// per level).  Min and max are exact, so the boxes equal the per-depth ones.
+constexpr int kPyrNT = 256;
+constexpr int kPyrSpan = 9;                         // levels per pass
+constexpr uint32_t kPyrBase = 2 * kPyrNT;           // base entries per block
+
+__device__ inline uint32_t sys_leaf_count(const TreeBuffers& tb, uint32_t M, uint32_t s)
+{
+    return ((s + 1 < tb.n_sys) ? tb.sys[s + 1].leaf_lo : M) - tb.sys[s].leaf_lo;
+}
+
+// ceil(ns / 2^k), ns >= 1
+__device__ inline uint32_t pyr_count(uint32_t ns, int k) { return k >= 32 ? 1u : ((ns - 1u) >> k) + 1u; }
+
+// first entry of system s at level k (1..kPyrLevels) in tb.pyr
+__device__ inline size_t pyr_sys_off(const TreeBuffers& tb, uint32_t M, uint32_t s, int k)
+{
+    size_t o = pyr_level_off(tb.pyr_cap, k);
+    for (uint32_t q = 0; q < s; ++q) o += pyr_count(sys_leaf_count(tb, M, q), k);
+    return o;
+}
+
+__device__ inline void pyr_load_leaf(const TreeBuffers& tb, size_t j, double* v)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = tb.lraw[4 * j + q];
+    v[4] = tb.lsec[j];
+    v[5] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[6 + q] = tb.lbox[6 * j + q];
+}
+
+__device__ inline void pyr_load(const TreeBuffers& tb, size_t e, double* v)
+{
+    const double2* p = reinterpret_cast<const double2*>(tb.pyr + kPyrDoubles * e);
+#pragma unroll
+    for (int q = 0; q < kPyrDoubles / 2; ++q) {
+        const double2 d = p[q];
+        v[2 * q] = d.x;
+        v[2 * q + 1] = d.y;
+    }
+}
+
+__device__ inline void pyr_store(const TreeBuffers& tb, size_t e, const double* v)
+{
+    double2* p = reinterpret_cast<double2*>(tb.pyr + kPyrDoubles * e);
+#pragma unroll
+    for (int q = 0; q < kPyrDoubles / 2; ++q) p[q] = make_double2(v[2 * q], v[2 * q + 1]);
+}
+
+// a = a (+) b: the sums left to right, the box by min and max
+__device__ inline void pyr_combine(double* a, const double* b)
+{
+#pragma unroll
+    for (int q = 0; q < 5; ++q) a[q] = a[q] + b[q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        a[6 + q] = dmin(a[6 + q], b[6 + q]);
+        a[9 + q] = dmax(a[9 + q], b[9 + q]);
+    }
+}
+
+// Levels kb + 1 .. kb + 9 from level kb (kb = 0: the leaves): block b takes
+// 512 base entries of one system (the systems' tiles in system order).
+__global__ __launch_bounds__(kPyrNT) void k_pyr_pass(TreeBuffers tb, int kb)
+{
+    __shared__ double e[kPyrNT][kPyrDoubles];
+    __shared__ uint32_t s_sys, s_tile;
+    __shared__ size_t s_off[kPyrSpan + 1];
+    const uint32_t M = tb.sc->n_leaves;
+    if (M == 0) return;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        s_sys = 0xffffffffu;
+        uint32_t first = 0;
+        for (uint32_t q = 0; q < tb.n_sys; ++q) {
+            const uint32_t tiles = (pyr_count(sys_leaf_count(tb, M, q), kb) + kPyrBase - 1) / kPyrBase;
+            if (blockIdx.x < first + tiles) {
+                s_sys = q;
+                s_tile = blockIdx.x - first;
+                break;
+            }
+            first += tiles;
+        }
+    }
+    // the first pass's block 0 also writes every system's level offsets for k_pyr_nodes
+    if (kb == 0 && blockIdx.x == 0)
+        for (int k = 1 + t; k <= kPyrLevels; k += kPyrNT) {
+            size_t o = pyr_level_off(tb.pyr_cap, k);
+            for (uint32_t q = 0; q < tb.n_sys; ++q) {
+                tb.sys[q].pyr_off[k] = static_cast<uint32_t>(o);
+                o += pyr_count(sys_leaf_count(tb, M, q), k);
+            }
+        }
+    __syncthreads();
+    const uint32_t sy = s_sys, c = s_tile;
+    if (sy == 0xffffffffu) return;
+    if (t <= kPyrSpan && kb + t >= 1 && kb + t <= kPyrLevels) s_off[t] = pyr_sys_off(tb, M, sy, kb + t);
+    __syncthreads();
+    const uint32_t ns = sys_leaf_count(tb, M, sy), L0 = tb.sys[sy].leaf_lo;
+    const uint32_t nb = pyr_count(ns, kb);
+    // level kb + 1: entry 256 c + t from base entries 512 c + 2t and 512 c + 2t + 1
+    double v[kPyrDoubles];
+    const uint32_t g0 = kPyrBase * c + 2u * t;
+    const bool valid = g0 < nb;
+    if (valid) {
+        if (kb == 0) pyr_load_leaf(tb, L0 + g0, v);
+        else pyr_load(tb, s_off[0] + g0, v);
+        if (g0 + 1 < nb) {
+            double w[kPyrDoubles];
+            if (kb == 0) pyr_load_leaf(tb, L0 + g0 + 1, w);
+            else pyr_load(tb, s_off[0] + g0 + 1, w);
+            pyr_combine(v, w);
+        }
+        pyr_store(tb, s_off[1] + (g0 >> 1), v);
+#pragma unroll
+        for (int q = 0; q < kPyrDoubles; ++q) e[t][q] = v[q];
+    }
+    for (int j = 2; j <= kPyrSpan && kb + j <= kPyrLevels; ++j) {
+        __syncthreads();
+        const uint32_t a = (kPyrBase * c >> j) + static_cast<uint32_t>(t);   // this level's entry
+        const bool ok = t < static_cast<int>(kPyrBase >> j) && a < pyr_count(ns, kb + j);
+        if (ok) {
+#pragma unroll
+            for (int q = 0; q < kPyrDoubles; ++q) v[q] = e[2 * t][q];
+            if (2 * a + 1 < pyr_count(ns, kb + j - 1)) {
+                double w[kPyrDoubles];
+#pragma unroll
+                for (int q = 0; q < kPyrDoubles; ++q) w[q] = e[2 * t + 1][q];
+                pyr_combine(v, w);
+            }
+        }
+        __syncthreads();
+        if (ok) {
+#pragma unroll
+            for (int q = 0; q < kPyrDoubles; ++q) e[t][q] = v[q];
+            pyr_store(tb, s_off[j] + a, v);
+        }
+    }
+}
+
+// The light moments and box of every branching node: its leaf range cut into
+// aligned pieces (the largest power of two that the start is a multiple of
+// and that fits), combined left to right, four pieces' loads at a time.
+__global__ __launch_bounds__(kPyrNT) void k_pyr_nodes(TreeBuffers tb)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kPyrNT + threadIdx.x;
+    const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
+    if (bi >= B) return;
+    const uint32_t lo = tb.blo[bi], hi = tb.bhi[bi];
+    const uint32_t sy = tb.n_sys == 1 ? 0u : static_cast<uint32_t>(tb.bsys[tb.leaf_first[lo]]);
+    const SysScalars* ys = tb.sys + sy;
+    const uint32_t L0 = ys->leaf_lo;
+    uint32_t r0 = lo - L0;
+    const uint32_t r1 = hi - L0 + 1;
+    double acc[kPyrDoubles];
+    bool first = true;
+    while (r0 < r1) {
+        constexpr int kBatch = 4;
+        double p[kBatch][kPyrDoubles];
+        bool have[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            have[q] = r0 < r1;
+            if (!have[q]) continue;
+            const int a = r0 == 0 ? 31 : __builtin_ctz(r0);
+            const int b = 31 - __builtin_clz(r1 - r0);
+            const int k = a < b ? a : b;
+            if (k == 0) pyr_load_leaf(tb, L0 + r0, p[q]);
+            else pyr_load(tb, ys->pyr_off[k] + (r0 >> k), p[q]);
+            r0 += 1u << k;
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            if (!have[q]) continue;
+            if (first) {
+#pragma unroll
+                for (int z = 0; z < kPyrDoubles; ++z) acc[z] = p[q][z];
+                first = false;
+            } else {
+                pyr_combine(acc, p[q]);
+            }
+        }
+    }
+    const size_t t = static_cast<size_t>(M) + bi;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) tb.lraw[4 * t + q] = acc[q];
+    tb.lsec[t] = acc[4];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) tb.lbox[6 * t + q] = acc[6 + q];
+}
+
 // reversed per-leaf node counts for the suffix scan
 __global__ __launch_bounds__(kNT) void k_suffix_input(size_t n, TreeBuffers tb, uint32_t* __restrict__ out)
 {
@@ -771,6 +972,15 @@ void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s)
     const int from = tb.deep_from < kMaxDepth ? tb.deep_from : kMaxDepth;
     if (from < kMaxDepth) k_moments_deep<<<1, kDeepNT, 0, s>>>(tb);
     for (int d = from - 1; d >= 0; --d) k_moments_level<<<nblk(n), kNT, 0, s>>>(tb, d);
+}
+
+void launch_light_moments(size_t n, TreeBuffers& tb, hipStream_t s)
+{
+    const unsigned sys = tb.n_sys + 1;
+    k_pyr_pass<<<static_cast<unsigned>(n / kPyrBase) + sys, kPyrNT, 0, s>>>(tb, 0);
+    k_pyr_pass<<<static_cast<unsigned>(n >> (kPyrSpan + 9)) + sys, kPyrNT, 0, s>>>(tb, kPyrSpan);
+    k_pyr_pass<<<sys, kPyrNT, 0, s>>>(tb, 2 * kPyrSpan);
+    k_pyr_nodes<<<nblk(n), kPyrNT, 0, s>>>(tb);
 }
 
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s)
