@@ -234,6 +234,8 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     if (!mixed) {
         tb.lraw = nullptr;
         tb.gflag = nullptr;
+    } else {
+        tb.deep_from = bh::kMaxDepth;   // the light-moment pyramid needs no list of deep nodes
     }
     bh::launch_build_tree(c->cur, c->n, tb, !mixed, s);
     if (mixed) {
