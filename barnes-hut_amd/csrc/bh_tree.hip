@@ -976,10 +976,13 @@ void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s)
 
 void launch_light_moments(size_t n, TreeBuffers& tb, hipStream_t s)
 {
+    // a node's pieces reach level floor(log2(leaves of its system)) <=
+    // floor(log2(n)), so a pass whose levels start above that is skipped
     const unsigned sys = tb.n_sys + 1;
     k_pyr_pass<<<static_cast<unsigned>(n / kPyrBase) + sys, kPyrNT, 0, s>>>(tb, 0);
-    k_pyr_pass<<<static_cast<unsigned>(n >> (kPyrSpan + 9)) + sys, kPyrNT, 0, s>>>(tb, kPyrSpan);
-    k_pyr_pass<<<sys, kPyrNT, 0, s>>>(tb, 2 * kPyrSpan);
+    if (n >> (kPyrSpan + 1))
+        k_pyr_pass<<<static_cast<unsigned>(n >> (kPyrSpan + 9)) + sys, kPyrNT, 0, s>>>(tb, kPyrSpan);
+    if (n >> (2 * kPyrSpan + 1)) k_pyr_pass<<<sys, kPyrNT, 0, s>>>(tb, 2 * kPyrSpan);
     k_pyr_nodes<<<nblk(n), kPyrNT, 0, s>>>(tb);
 }
 
