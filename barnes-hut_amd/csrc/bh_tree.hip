@@ -683,14 +683,6 @@ __device__ inline uint32_t sys_leaf_count(const TreeBuffers& tb, uint32_t M, uin
 // ceil(ns / 2^k), ns >= 1
 __device__ inline uint32_t pyr_count(uint32_t ns, int k) { return k >= 32 ? 1u : ((ns - 1u) >> k) + 1u; }
 
-// first entry of system s at level k (1..kPyrLevels) in tb.pyr
-__device__ inline size_t pyr_sys_off(const TreeBuffers& tb, uint32_t M, uint32_t s, int k)
-{
-    size_t o = pyr_level_off(tb.pyr_cap, k);
-    for (uint32_t q = 0; q < s; ++q) o += pyr_count(sys_leaf_count(tb, M, q), k);
-    return o;
-}
-
 __device__ inline void pyr_load_leaf(const TreeBuffers& tb, size_t j, double* v)
 {
 #pragma unroll
@@ -735,17 +727,20 @@ __device__ inline void pyr_combine(double* a, const double* b)
 // 512 base entries of one system (the systems' tiles in system order).
 __global__ __launch_bounds__(kPyrNT) void k_pyr_pass(TreeBuffers tb, int kb)
 {
-    __shared__ double e[kPyrNT][kPyrDoubles];
+    __shared__ double e[kPyrDoubles][kPyrNT];   // one row per value: a pair (2t, 2t + 1) is one 16-byte read
     __shared__ uint32_t s_sys, s_tile;
     __shared__ size_t s_off[kPyrSpan + 1];
+    __shared__ uint32_t s_ns[kMaxSystems];      // leaves per system
     const uint32_t M = tb.sc->n_leaves;
     if (M == 0) return;
     const int t = threadIdx.x;
+    for (uint32_t q = t; q < tb.n_sys; q += kPyrNT) s_ns[q] = sys_leaf_count(tb, M, q);
+    __syncthreads();
     if (t == 0) {
         s_sys = 0xffffffffu;
         uint32_t first = 0;
         for (uint32_t q = 0; q < tb.n_sys; ++q) {
-            const uint32_t tiles = (pyr_count(sys_leaf_count(tb, M, q), kb) + kPyrBase - 1) / kPyrBase;
+            const uint32_t tiles = (pyr_count(s_ns[q], kb) + kPyrBase - 1) / kPyrBase;
             if (blockIdx.x < first + tiles) {
                 s_sys = q;
                 s_tile = blockIdx.x - first;
@@ -760,15 +755,19 @@ __global__ __launch_bounds__(kPyrNT) void k_pyr_pass(TreeBuffers tb, int kb)
             size_t o = pyr_level_off(tb.pyr_cap, k);
             for (uint32_t q = 0; q < tb.n_sys; ++q) {
                 tb.sys[q].pyr_off[k] = static_cast<uint32_t>(o);
-                o += pyr_count(sys_leaf_count(tb, M, q), k);
+                o += pyr_count(s_ns[q], k);
             }
         }
     __syncthreads();
     const uint32_t sy = s_sys, c = s_tile;
     if (sy == 0xffffffffu) return;
-    if (t <= kPyrSpan && kb + t >= 1 && kb + t <= kPyrLevels) s_off[t] = pyr_sys_off(tb, M, sy, kb + t);
+    if (t <= kPyrSpan && kb + t >= 1 && kb + t <= kPyrLevels) {
+        size_t o = pyr_level_off(tb.pyr_cap, kb + t);
+        for (uint32_t q = 0; q < sy; ++q) o += pyr_count(s_ns[q], kb + t);
+        s_off[t] = o;
+    }
     __syncthreads();
-    const uint32_t ns = sys_leaf_count(tb, M, sy), L0 = tb.sys[sy].leaf_lo;
+    const uint32_t ns = s_ns[sy], L0 = tb.sys[sy].leaf_lo;
     const uint32_t nb = pyr_count(ns, kb);
     // level kb + 1: entry 256 c + t from base entries 512 c + 2t and 512 c + 2t + 1
     double v[kPyrDoubles];
@@ -785,26 +784,26 @@ __global__ __launch_bounds__(kPyrNT) void k_pyr_pass(TreeBuffers tb, int kb)
         }
         pyr_store(tb, s_off[1] + (g0 >> 1), v);
 #pragma unroll
-        for (int q = 0; q < kPyrDoubles; ++q) e[t][q] = v[q];
+        for (int q = 0; q < kPyrDoubles; ++q) e[q][t] = v[q];
     }
     for (int j = 2; j <= kPyrSpan && kb + j <= kPyrLevels; ++j) {
         __syncthreads();
         const uint32_t a = (kPyrBase * c >> j) + static_cast<uint32_t>(t);   // this level's entry
         const bool ok = t < static_cast<int>(kPyrBase >> j) && a < pyr_count(ns, kb + j);
         if (ok) {
+            double w[kPyrDoubles];
 #pragma unroll
-            for (int q = 0; q < kPyrDoubles; ++q) v[q] = e[2 * t][q];
-            if (2 * a + 1 < pyr_count(ns, kb + j - 1)) {
-                double w[kPyrDoubles];
-#pragma unroll
-                for (int q = 0; q < kPyrDoubles; ++q) w[q] = e[2 * t + 1][q];
-                pyr_combine(v, w);
+            for (int q = 0; q < kPyrDoubles; ++q) {
+                const double2 d = *reinterpret_cast<const double2*>(&e[q][2 * t]);
+                v[q] = d.x;
+                w[q] = d.y;
             }
+            if (2 * a + 1 < pyr_count(ns, kb + j - 1)) pyr_combine(v, w);
         }
         __syncthreads();
         if (ok) {
 #pragma unroll
-            for (int q = 0; q < kPyrDoubles; ++q) e[t][q] = v[q];
+            for (int q = 0; q < kPyrDoubles; ++q) e[q][t] = v[q];
             pyr_store(tb, s_off[j] + a, v);
         }
     }
