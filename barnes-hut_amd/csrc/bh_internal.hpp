@@ -257,9 +257,10 @@ void launch_keys(const Bodies& b, size_t n, const TreeBuffers& tb, const double*
 // system also starts a leaf); heavy slots of every system.
 void launch_permute(const Bodies& src, const Bodies& dst, const TreeBuffers& tb, size_t n, hipStream_t s);
 // Tree topology and raw moments from sorted keys / bodies and first-of-key
-// flags (tb.flag); light moments too when tb.lraw is set.  moments = false
-// stops after the topology and the leaf moments; launch_tree_moments then
-// adds the branching nodes' moments (21 per-depth launches).
+// flags (tb.flag); the leaves' light moments instead when tb.lraw is set (the
+// mixed forms).  moments = false stops after the topology and the leaf
+// moments; launch_tree_moments then adds the branching nodes' raw moments
+// (one launch per populated depth, the plain form's order).
 void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments, hipStream_t s);
 void launch_tree_moments(size_t n, TreeBuffers& tb, hipStream_t s);
 // the mixed forms' light moments and boxes of the branching nodes, from the

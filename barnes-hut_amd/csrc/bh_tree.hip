@@ -531,11 +531,10 @@ __global__ __launch_bounds__(kNT) void k_link(size_t n, TreeBuffers tb)
 }
 
 // Moments of branching node bi from its existing children, summed in octant
-// order 0..7 from 0.0 (src/linear_octree.cpp:69-87): the raw moments for the
-// plain form, or, when the mixed forms set tb.lraw, the light moments and
-// light-body boxes (the mixed walk never reads the raw moments).  Every
-// child's values are loaded before the first sum (an empty octant loads the
-// node's first child again and is then skipped), so one node costs two
+// order 0..7 from 0.0 (src/linear_octree.cpp:69-87): the plain form's raw
+// moments (the mixed forms take their light moments from the pyramid below).
+// Every child's values are loaded before the first sum (an empty octant loads
+// the node's first child again and is then skipped), so one node costs two
 // dependent memory round trips, not one per child.
 __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi)
 {
@@ -548,58 +547,19 @@ __device__ inline void node_moments(const TreeBuffers& tb, uint32_t M, size_t bi
     size_t ix[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) ix[k] = static_cast<size_t>(c[k] < 0 ? any : c[k]);
-    if (!tb.lraw) {
-        double4 r[8];
+    double4 r[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) r[k] = reinterpret_cast<const double4*>(tb.raw)[ix[k]];
-        double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < 8; ++k) r[k] = reinterpret_cast<const double4*>(tb.raw)[ix[k]];
+    double sm = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (c[k] < 0) continue;
-            sm += r[k].x;
-            sx += r[k].y;
-            sy += r[k].z;
-            sz += r[k].w;
-        }
-        reinterpret_cast<double4*>(tb.raw)[static_cast<size_t>(M) + bi] = make_double4(sm, sx, sy, sz);
-    } else {
-        double4 q[8];
-        double sec[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            q[k] = reinterpret_cast<const double4*>(tb.lraw)[ix[k]];
-            sec[k] = tb.lsec[ix[k]];
-        }
-        double lm = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, ls = 0.0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (c[k] < 0) continue;
-            lm += q[k].x;
-            lx += q[k].y;
-            ly += q[k].z;
-            lz += q[k].w;
-            ls += sec[k];
-        }
-        double bx[6];
-#pragma unroll
-        for (int a = 0; a < 6; ++a) {
-            double cbv[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) cbv[k] = tb.lbox[6 * ix[k] + a];
-            double v = (a < 3) ? INFINITY : -INFINITY;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (c[k] < 0) continue;
-                v = (a < 3) ? dmin(v, cbv[k]) : dmax(v, cbv[k]);
-            }
-            bx[a] = v;
-        }
-        double* ob = tb.lbox + 6 * (static_cast<size_t>(M) + bi);
-#pragma unroll
-        for (int a = 0; a < 6; ++a) ob[a] = bx[a];
-        reinterpret_cast<double4*>(tb.lraw)[static_cast<size_t>(M) + bi] = make_double4(lm, lx, ly, lz);
-        tb.lsec[static_cast<size_t>(M) + bi] = ls;
+    for (int k = 0; k < 8; ++k) {
+        if (c[k] < 0) continue;
+        sm += r[k].x;
+        sx += r[k].y;
+        sy += r[k].z;
+        sz += r[k].w;
     }
+    reinterpret_cast<double4*>(tb.raw)[static_cast<size_t>(M) + bi] = make_double4(sm, sx, sy, sz);
 }
 
 // raw moments of the branching nodes of one depth; levels run bottom-up, one
