@@ -111,15 +111,22 @@ void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size
 
 // Block-tile scans fused into their producer and consumer kernels: the
 // kernel that makes one value per thread (blocks of 256) leaves each block's
-// sum (scan_block_sum), scan_tile_offsets turns the sums into exclusive
-// offsets in place (one block; *total gets the sum), and the kernel that needs
-// the exclusive prefix adds its block's offset to the block-local scan
-// (scan_block_excl).  Every thread of the block calls the block helpers.
-// tmp holds tile_sum_words(n) u32.
+// sum (scan_block_sum), and the kernel that needs the exclusive prefix adds
+// its block's offset to the block-local scan (scan_block_excl).  Up to
+// kDirectTiles tiles the consumer's blocks add the sums of the tiles before
+// them themselves (at most 16 loads a thread, from L2), so no launch runs in
+// between; above that, scan_tile_offsets turns the sums into exclusive
+// offsets in place first (one block).  Either way the consumer writes the
+// total (its thread n - 1 holds it).  Every thread of the block calls the
+// block helpers.  tmp holds tile_sum_words(n) u32.
 inline size_t tile_sum_words(size_t n) { return (n + 255) / 256 + 1; }
-void scan_tile_offsets(uint32_t* tsum, size_t n, uint32_t* total, hipStream_t s);
-// out[i] = exclusive prefix of in[i] (blocks of 256, offsets from scan_tile_offsets)
-void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* toff, uint32_t* out, hipStream_t s);
+constexpr size_t kDirectTiles = 4096;
+__host__ __device__ inline bool scan_direct(size_t n) { return (n + 255) / 256 <= kDirectTiles; }
+// launches nothing when scan_direct(n)
+void scan_tile_offsets(uint32_t* tsum, size_t n, hipStream_t s);
+// out[i] = exclusive prefix of in[i] (blocks of 256); *total = the sum
+void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* tsum, uint32_t* out, uint32_t* total,
+                     hipStream_t s);
 // tsum[block] = sum of in over the block's 256 values
 void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s);
 
@@ -132,10 +139,20 @@ __device__ inline void scan_block_sum(uint32_t v, uint32_t* tsum)
     if (threadIdx.x == 0) tsum[blockIdx.x] = w4[0] + w4[1] + w4[2] + w4[3];
 }
 
-__device__ inline uint32_t scan_block_excl(uint32_t v, const uint32_t* toff)
+// n: the number of values scanned (it selects where the block offset comes from)
+__device__ inline uint32_t scan_block_excl(uint32_t v, const uint32_t* tsum, size_t n)
 {
-    __shared__ uint32_t w4[4];
+    __shared__ uint32_t w4[4], p4[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool direct = scan_direct(n);
+    if (direct) {
+        const uint32_t nt = static_cast<uint32_t>((n + 255) / 256);
+        const uint32_t lim = blockIdx.x < nt ? blockIdx.x : nt;
+        uint32_t pre = 0;
+        for (uint32_t k = threadIdx.x; k < lim; k += 256) pre += tsum[k];
+        for (int o = 32; o > 0; o >>= 1) pre += static_cast<uint32_t>(__shfl_xor(static_cast<int>(pre), o, 64));
+        if (lane == 0) p4[w] = pre;
+    }
     uint32_t inc = v;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc), o, 64));
@@ -143,7 +160,7 @@ __device__ inline uint32_t scan_block_excl(uint32_t v, const uint32_t* toff)
     }
     if (lane == 63) w4[w] = inc;
     __syncthreads();
-    uint32_t off = toff[blockIdx.x];
+    uint32_t off = direct ? p4[0] + p4[1] + p4[2] + p4[3] : tsum[blockIdx.x];
     for (int q = 0; q < w; ++q) off += w4[q];
     return off + inc - v;
 }

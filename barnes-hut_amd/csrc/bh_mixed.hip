@@ -269,8 +269,12 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
 
 __device__ inline void group_first(size_t i, size_t n, uint32_t gid, const TreeBuffers& tb, const MixedBuffers& mb)
 {
-    if (tb.gflag[i]) mb.gfirst[gid] = static_cast<uint32_t>(i);
-    if (i == 0) mb.gfirst[tb.sc->n_groups] = static_cast<uint32_t>(n);
+    const uint32_t f = tb.gflag[i];
+    if (f) mb.gfirst[gid] = static_cast<uint32_t>(i);
+    if (i + 1 == n) {   // the group count G and the end sentinel gfirst[G] = n
+        mb.gfirst[gid + f] = static_cast<uint32_t>(n);
+        tb.sc->n_groups = gid + f;
+    }
 }
 
 // number of children of every branching node (into base, scanned next)
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, G
 __global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb)
 {
     const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
-    const uint32_t base = scan_block_excl(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp);
+    const uint32_t base = scan_block_excl(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp, n);
     if (bi < n) mb.base[bi] = base;
     if (bi < tb.n_sys) mb.slot[tb.sys[bi].root] = static_cast<uint32_t>(bi);
     if (bi >= tb.sc->n_branch) return;
@@ -316,7 +320,7 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuf
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (static_cast<size_t>(blockIdx.x) * kNT < n) {   // block-uniform: the blocks over the bodies
-        const uint32_t gid = scan_block_excl(t < n ? tb.gflag[t] : 0u, tb.scan_tmp);
+        const uint32_t gid = scan_block_excl(t < n ? tb.gflag[t] : 0u, tb.scan_tmp, n);
         if (t < n) group_first(t, n, gid, tb, mb);
     }
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
@@ -1165,7 +1169,7 @@ void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const Grou
 {
     k_split_hist<<<hist_blocks(n), kNT, 0, s>>>(tb);
     k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, gp);
-    scan_tile_offsets(tb.scan_tmp, n, nullptr, s);
+    scan_tile_offsets(tb.scan_tmp, n, s);
     k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
 }
 
@@ -1176,7 +1180,7 @@ void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffer
     // group index of every first body: the group flags scanned, the block
     // sums here, the block-local scan in k_mixed_nodes
     scan_tile_sums256(tb.gflag, n, tb.scan_tmp, s);
-    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_groups, s);
+    scan_tile_offsets(tb.scan_tmp, n, s);
     k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, b, tb, mb, G);
 }
 

@@ -268,11 +268,14 @@ __global__ __launch_bounds__(256) void k_tile_sums256(const uint32_t* __restrict
 }
 
 __global__ __launch_bounds__(256) void k_tile_apply256(const uint32_t* __restrict__ in, size_t n,
-                                                       const uint32_t* __restrict__ toff, uint32_t* __restrict__ out)
+                                                       const uint32_t* __restrict__ tsum, uint32_t* __restrict__ out,
+                                                       uint32_t* __restrict__ total)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    const uint32_t e = scan_block_excl(i < n ? in[i] : 0u, toff);
+    const uint32_t v = i < n ? in[i] : 0u;
+    const uint32_t e = scan_block_excl(v, tsum, n);
     if (i < n) out[i] = e;
+    if (i + 1 == n) *total = e + v;
 }
 
 // in and out may alias: a tile reads its values before it writes them
@@ -1151,16 +1154,18 @@ __global__ __launch_bounds__(kNT) void k_ns_merge(uint64_t* __restrict__ k0, uin
 
 size_t scan_tmp_words(size_t cap) { return std::max(scan_tiles(cap) + 1, tile_sum_words(cap)); }
 
-void scan_tile_offsets(uint32_t* tsum, size_t n, uint32_t* total, hipStream_t s)
+void scan_tile_offsets(uint32_t* tsum, size_t n, hipStream_t s)
 {
+    if (scan_direct(n)) return;
     const size_t nt = (n + 255) / 256;
-    if (nt <= static_cast<size_t>(kTile)) k_scan_sums<<<1, kNT, 0, s>>>(tsum, nt, total);
-    else k_scan_sums_wide<<<1, kWideNT, 0, s>>>(tsum, nt, total);
+    if (nt <= static_cast<size_t>(kTile)) k_scan_sums<<<1, kNT, 0, s>>>(tsum, nt, nullptr);
+    else k_scan_sums_wide<<<1, kWideNT, 0, s>>>(tsum, nt, nullptr);
 }
 
-void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* toff, uint32_t* out, hipStream_t s)
+void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* tsum, uint32_t* out, uint32_t* total,
+                     hipStream_t s)
 {
-    k_tile_apply256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, toff, out);
+    k_tile_apply256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, tsum, out, total);
 }
 
 void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s)

@@ -359,14 +359,15 @@ __global__ __launch_bounds__(kNT) void k_permute(Bodies src, Bodies dst, TreeBuf
 }
 
 // ------------------------------------------------------------- tree
-// leaves: distinct keys, first body of each leaf, leaf id of each body
-// (the leaf index: the scan of the flags, offsets from scan_tile_offsets)
+// leaves: distinct keys, first body of each leaf, leaf id of each body (the
+// leaf index: the scan of the flags); the last body's thread sets the leaf
+// count M and the end sentinel leaf_first[M] = n
 __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key, const uint32_t* __restrict__ flag,
                                                 size_t n, TreeBuffers tb)
 {
     const size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     const uint32_t f = i < n ? flag[i] : 0u;
-    const uint32_t ex = scan_block_excl(f, tb.scan_tmp);
+    const uint32_t ex = scan_block_excl(f, tb.scan_tmp, n);
     if (i >= n) return;
     const uint32_t j = ex + f - 1;
     tb.leaf_of[i] = j;
@@ -377,7 +378,10 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
         SysScalars* sy = tb.sys + tb.bsys[i];
         if (i == sy->body_lo) sy->leaf_lo = j;
     }
-    if (i == 0) tb.leaf_first[tb.sc->n_leaves] = static_cast<uint32_t>(n);
+    if (i + 1 == n) {
+        tb.leaf_first[j + 1] = static_cast<uint32_t>(n);
+        tb.sc->n_leaves = j + 1;
+    }
 }
 
 // leaf moments (raw, or light moments and boxes for the mixed forms);
@@ -950,11 +954,11 @@ void launch_build_tree(const Bodies& b, size_t n, TreeBuffers& tb, bool moments,
     // leaf index of every body: exclusive scan of first-of-key flags (M total)
     // leaf index of every body: the first-of-key flags scanned (k_permute
     // left the block sums); M = n_leaves
-    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_leaves, s);
+    scan_tile_offsets(tb.scan_tmp, n, s);
     k_leaves<<<nblk(n), kNT, 0, s>>>(tb.key, tb.flag, n, tb);
     k_leaf_moments_bounds<<<nblk(n), kNT, 0, s>>>(b, n, tb);
-    scan_tile_offsets(tb.scan_tmp, n, &tb.sc->n_branch, s);
-    scan_tile_apply(tb.is_rep, n, tb.scan_tmp, tb.bidx, s);
+    scan_tile_offsets(tb.scan_tmp, n, s);
+    scan_tile_apply(tb.is_rep, n, tb.scan_tmp, tb.bidx, &tb.sc->n_branch, s);
     k_branch_init<<<nblk(n), kNT, 0, s>>>(n, tb);
     k_link<<<nblk(2 * n), kNT, 0, s>>>(n, tb);
     if (moments) launch_tree_moments(n, tb, s);
