@@ -173,7 +173,7 @@ int reserve(bh_ctx* c, size_t n)
     A(t.bidx, N); A(t.cnt_hi, N); A(t.suffix, N + 1);
     A(t.bdepth, N); A(t.blo, N); A(t.bhi, N); A(t.child, 8 * N);
     A(t.parent, 2 * N); A(t.windex, 2 * N); A(t.raw, 8 * N); A(t.lraw, 8 * N); A(t.lbox, 12 * N); A(t.lsec, 2 * N); A(t.walk, 2 * N);
-    A(t.gflag, N); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems); A(t.deep, N);
+    A(t.gflag, N); A(t.gtile, bh::tile_sum_words(N)); A(c->bsys_w, N); A(t.sys, bh::kMaxSystems); A(t.deep, N);
     A(t.pyr, bh::kPyrDoubles * bh::pyr_entries(N));
     t.pyr_cap = static_cast<uint32_t>(N);
     t.bsys = c->bsys_w;
@@ -234,6 +234,7 @@ int run_force(bh_ctx* c, int form, double theta, double G, double soft2, bool ki
     if (!mixed) {
         tb.lraw = nullptr;
         tb.gflag = nullptr;
+        tb.gtile = nullptr;
     } else {
         tb.deep_from = bh::kMaxDepth;   // the light-moment pyramid needs no list of deep nodes
     }

@@ -127,8 +127,6 @@ void scan_tile_offsets(uint32_t* tsum, size_t n, hipStream_t s);
 // out[i] = exclusive prefix of in[i] (blocks of 256); *total = the sum
 void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* tsum, uint32_t* out, uint32_t* total,
                      hipStream_t s);
-// tsum[block] = sum of in over the block's 256 values
-void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s);
 
 __device__ inline void scan_block_sum(uint32_t v, uint32_t* tsum)
 {
@@ -229,6 +227,7 @@ struct TreeBuffers {
     double* lbox;         // 6 per node: bounding box of the light bodies (min xyz, max xyz)
     double* lsec;         // per node: sum of m |x|^2 over the light bodies (mixed forms)
     uint32_t* gflag;      // walk-group start flags (mixed forms), cleared per step
+    uint32_t* gtile;      // flags set per 256-body tile (the group scan's tile sums), cleared per step
     WalkNode* walk;       // capacity 2N
     uint32_t* scan_tmp;
     uint32_t* sort_tmp;

@@ -74,16 +74,16 @@ __device__ inline uint32_t enc(uint32_t first, uint32_t count, bool body)
 // holds 1 + sum_{e < d} dcount[e] non-empty cells of s.  With one system each
 // wavefront counts 64 boundaries per round with one ballot per depth; with
 // several, LDS counters per (system, depth).  Blocks add their counts into
-// dcount, which sys_extents (block 0 of k_child_count) reads and clears.
-// Blocks: one per 4,096 bodies, at least 128 and at most 2,048, so each
-// wavefront takes about 16 rounds of 64 boundaries (with 128 blocks at 4.1M
-// bodies each took 126 rounds of dependent loads: 261 us).
+// dcount, which sys_extents (block 0 of k_slots) reads and clears.
+// Histogram blocks: one per 4,096 bodies, at least 128 and at most 2,048, so
+// each wavefront takes about 16 rounds of 64 boundaries (with 128 blocks at
+// 4.1M bodies each took 126 rounds of dependent loads: 261 us).
 inline unsigned hist_blocks(size_t n)
 {
     return static_cast<unsigned>(std::min<size_t>(2048, std::max<size_t>(128, n / 4096)));
 }
 
-__global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
+__device__ inline void split_hist(const TreeBuffers& tb, uint32_t hist_grid)
 {
     __shared__ uint32_t h[kMaxSystems][kMaxDepth + 1];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
     const uint32_t rows = tb.n_sys;
     for (uint32_t k = threadIdx.x; k < rows * (kMaxDepth + 1); k += kNT) (&h[0][0])[k] = 0;
     __syncthreads();
-    const uint32_t stride = gridDim.x * kNT;
+    const uint32_t stride = hist_grid * kNT;
     if (rows == 1) {
         uint32_t cnt[kMaxDepth + 1];
 #pragma unroll
@@ -132,12 +132,35 @@ __global__ __launch_bounds__(kNT) void k_split_hist(TreeBuffers tb)
     }
 }
 
+// number of children of every branching node (into base, scanned next)
+__device__ inline uint32_t child_count(const TreeBuffers& tb, size_t bi)
+{
+    uint32_t c = 0;
+    if (bi < tb.sc->n_branch) {
+        const int32_t* ch = tb.child + 8 * bi;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
+    }
+    return c;
+}
+
+// The first hist_grid blocks count the split depths; every block over the
+// branching nodes also leaves the block sum of their child counts (k_slots
+// scans them).
+__global__ __launch_bounds__(kNT) void k_split_hist(size_t n, TreeBuffers tb, uint32_t hist_grid)
+{
+    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    const uint32_t cc = bi < n ? child_count(tb, bi) : 0u;
+    if (blockIdx.x < hist_grid) split_hist(tb, hist_grid);
+    if (static_cast<size_t>(blockIdx.x) * kNT < n) scan_block_sum(cc, tb.scan_tmp);   // block-uniform
+}
+
 // Largest box extent a walk group of system s may have: efac x the cell side
 // at the first depth whose non-empty cells number at least n_s / occ (with
 // occ = 32, cells there hold ~32 bodies on average).  A group much wider than
 // its neighbours' spacing would force the walk to open everything its box
-// overlaps.  Clears dcount for the next step.  Run by block 0 of
-// k_child_count, the launch after k_split_hist.
+// overlaps.  Clears dcount for the next step.  Run by block 0 of k_slots,
+// the launch after k_split_hist.
 __device__ inline void sys_extents(const TreeBuffers& tb, const GroupParams& gp)
 {
     for (uint32_t q = threadIdx.x; q < tb.n_sys; q += kNT) {
@@ -164,6 +187,15 @@ __device__ inline double box_extent(const double* bx)
 // packed while the count fits gmax and their joint box stays within the
 // limit.  Everything here depends on the system's bodies alone, so every
 // schedule, and every other system of an ensemble, sees the same groups.
+// Body i starts a group: its flag, counted in its 256-body tile.  Each body
+// is marked by one node at most (the root, the co-keyed leaf that holds it,
+// or the parent of the unit it starts), so the count is the tile's flag sum.
+__device__ inline void mark_group(const TreeBuffers& tb, uint32_t i)
+{
+    tb.gflag[i] = 1;
+    atomicAdd(tb.gtile + (i >> 8), 1u);
+}
+
 __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, MixedBuffers mb, uint32_t gmax)
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
@@ -193,14 +225,14 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
     };
     const bool ugood = good_box(leaf, cnt, ub);
     if (par == kNone && ugood) {
-        tb.gflag[f] = 1;
+        mark_group(tb, f);
         mb.gdepth[f] = static_cast<uint8_t>(depth);
         return;
     }
     if (leaf) {
         if (cnt > gmax) {   // co-keyed bodies beyond one group
             for (uint32_t k = 0; k < cnt; k += gmax) {
-                tb.gflag[f + k] = 1;
+                mark_group(tb, f + k);
                 mb.gdepth[f + k] = kMaxDepth;
             }
         }
@@ -249,7 +281,7 @@ __global__ __launch_bounds__(kNT) void k_group_marks(size_t n, TreeBuffers tb, M
             fits = e <= emax;
         }
         if (!fits) {
-            tb.gflag[cf[k]] = 1;
+            mark_group(tb, cf[k]);
             mb.gdepth[cf[k]] = static_cast<uint8_t>(depth + 1);
             cur = 0;
             open = true;
@@ -277,32 +309,13 @@ __device__ inline void group_first(size_t i, size_t n, uint32_t gid, const TreeB
     }
 }
 
-// number of children of every branching node (into base, scanned next)
-__device__ inline uint32_t child_count(const TreeBuffers& tb, size_t bi)
-{
-    uint32_t c = 0;
-    if (bi < tb.sc->n_branch) {
-        const int32_t* ch = tb.child + 8 * bi;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
-    }
-    return c;
-}
-
-// block sums of the child counts (k_slots scans them); block 0 also sets the
-// systems' extent limits
-__global__ __launch_bounds__(kNT) void k_child_count(size_t n, TreeBuffers tb, GroupParams gp)
+// slots: the root of system s at s; children of branching node b at
+// n_sys + base[b] + rank, base = the exclusive scan of the child counts.
+// Block 0 also sets the systems' extent limits.
+__global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb, GroupParams gp)
 {
     const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (blockIdx.x == 0) sys_extents(tb, gp);
-    scan_block_sum(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp);
-}
-
-// slots: the root of system s at s; children of branching node b at
-// n_sys + base[b] + rank, base = the exclusive scan of the child counts
-__global__ __launch_bounds__(kNT) void k_slots(size_t n, TreeBuffers tb, MixedBuffers mb)
-{
-    const size_t bi = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     const uint32_t base = scan_block_excl(bi < n ? child_count(tb, bi) : 0u, tb.scan_tmp, n);
     if (bi < n) mb.base[bi] = base;
     if (bi < tb.n_sys) mb.slot[tb.sys[bi].root] = static_cast<uint32_t>(bi);
@@ -320,7 +333,7 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuf
 {
     const size_t t = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
     if (static_cast<size_t>(blockIdx.x) * kNT < n) {   // block-uniform: the blocks over the bodies
-        const uint32_t gid = scan_block_excl(t < n ? tb.gflag[t] : 0u, tb.scan_tmp, n);
+        const uint32_t gid = scan_block_excl(t < n ? tb.gflag[t] : 0u, tb.gtile, n);
         if (t < n) group_first(t, n, gid, tb, mb);
     }
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
@@ -1018,7 +1031,19 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
     const uint32_t h = sy->heavy_slot[k];
     Neumaier acc;
     const size_t first = sy->heavy_item + static_cast<size_t>(k) * sy->heavy_chunks;
-    for (uint32_t c = static_cast<uint32_t>(lane); c < sy->heavy_chunks; c += 64) acc.add(mb.hpart[3 * (first + c) + a]);
+    const uint32_t nc = sy->heavy_chunks;
+    // eight partials loaded per round before they are added (in chunk order)
+    for (uint32_t c0 = static_cast<uint32_t>(lane); c0 < nc; c0 += 64 * 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t c = c0 + 64 * j;
+            v[j] = c < nc ? mb.hpart[3 * (first + c) + a] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (c0 + 64 * j < nc) acc.add(v[j]);
+    }
     lanes[w][lane] = acc.value();
     __builtin_amdgcn_wave_barrier();
     if (lane != 0) return;
@@ -1167,20 +1192,19 @@ void launch_chunks(const Bodies& b, size_t n, const TreeBuffers& tb, const Mixed
 
 void launch_mixed_layout(size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp, hipStream_t s)
 {
-    k_split_hist<<<hist_blocks(n), kNT, 0, s>>>(tb);
-    k_child_count<<<nblk(n), kNT, 0, s>>>(n, tb, gp);
+    const unsigned hg = hist_blocks(n);
+    k_split_hist<<<std::max(hg, nblk(n)), kNT, 0, s>>>(n, tb, hg);
     scan_tile_offsets(tb.scan_tmp, n, s);
-    k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb);
+    k_slots<<<nblk(n), kNT, 0, s>>>(n, tb, mb, gp);
 }
 
 void launch_mixed_groups(const Bodies& b, size_t n, TreeBuffers& tb, MixedBuffers& mb, const GroupParams& gp,
                          double G, hipStream_t s)
 {
     k_group_marks<<<nblk(2 * n), kNT, 0, s>>>(n, tb, mb, gp.gmax < 1 ? 1u : (gp.gmax > 64 ? 64u : gp.gmax));
-    // group index of every first body: the group flags scanned, the block
-    // sums here, the block-local scan in k_mixed_nodes
-    scan_tile_sums256(tb.gflag, n, tb.scan_tmp, s);
-    scan_tile_offsets(tb.scan_tmp, n, s);
+    // group index of every first body: the group flags scanned, the tile
+    // sums counted by k_group_marks, the block-local scan in k_mixed_nodes
+    scan_tile_offsets(tb.gtile, n, s);
     k_mixed_nodes<<<nblk(2 * n), kNT, 0, s>>>(n, b, tb, mb, G);
 }
 

@@ -260,13 +260,6 @@ __global__ __launch_bounds__(kWideNT) void k_scan_sums_wide(uint32_t* __restrict
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-__global__ __launch_bounds__(256) void k_tile_sums256(const uint32_t* __restrict__ in, size_t n,
-                                                      uint32_t* __restrict__ tsum)
-{
-    const size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
-    scan_block_sum(i < n ? in[i] : 0u, tsum);
-}
-
 __global__ __launch_bounds__(256) void k_tile_apply256(const uint32_t* __restrict__ in, size_t n,
                                                        const uint32_t* __restrict__ tsum, uint32_t* __restrict__ out,
                                                        uint32_t* __restrict__ total)
@@ -1166,11 +1159,6 @@ void scan_tile_apply(const uint32_t* in, size_t n, const uint32_t* tsum, uint32_
                      hipStream_t s)
 {
     k_tile_apply256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, tsum, out, total);
-}
-
-void scan_tile_sums256(const uint32_t* in, size_t n, uint32_t* tsum, hipStream_t s)
-{
-    k_tile_sums256<<<static_cast<unsigned>((n + 255) / 256), 256, 0, s>>>(in, n, tsum);
 }
 
 void scan_exclusive_u32(const uint32_t* in, uint32_t* out, uint32_t* total, size_t n, uint32_t* tmp,

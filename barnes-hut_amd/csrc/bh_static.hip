@@ -114,6 +114,7 @@ StaticLayout carve(unsigned char* base, size_t cap)
     t.lbox = nullptr;
     t.lsec = nullptr;
     t.gflag = nullptr;
+    t.gtile = nullptr;
     t.key_alt2 = nullptr;
     t.perm_alt2 = nullptr;
     t.n_sys = 1;

@@ -371,7 +371,10 @@ __global__ __launch_bounds__(kNT) void k_leaves(const uint64_t* __restrict__ key
     if (i >= n) return;
     const uint32_t j = ex + f - 1;
     tb.leaf_of[i] = j;
-    if (tb.gflag) tb.gflag[i] = 0;
+    if (tb.gflag) {
+        tb.gflag[i] = 0;
+        if ((i & 255) == 0) tb.gtile[i >> 8] = 0;
+    }
     if (f) {
         tb.ukey[j] = key[i];
         tb.leaf_first[j] = static_cast<uint32_t>(i);
