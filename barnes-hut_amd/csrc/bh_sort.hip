@@ -829,11 +829,13 @@ __device__ inline uint32_t merge_rounds(uint32_t m)
 // Tile sort: input (k2, v2) in near mode, (k0, v0) in full mode; output (k1, v1).
 // A bitonic network over the tile, 66 phases.  Thread t holds elements
 // 2t and 2t + 1 in registers: the phases of partner distance 1 exchange in
-// registers, distances 2..64 across lanes by shuffles, and only distances
-// 128..1024 (10 phases) through LDS with barriers.  (Measured at 300k
-// bodies: every phase through LDS, 1024 threads, 35 us; 256 threads with
-// four exchanges each 80 us; 8192-key tiles held in registers 144 us -- the
-// few tiles of a small count leave the work to a few CUs.)
+// registers, distances 2..64 across lanes (DPP and swizzles, a permute for
+// 64), and only distances 128..1024 (10 phases) through LDS with barriers.
+// (Measured at 300k bodies: every phase through LDS, 1024 threads, 35 us;
+// 256 threads with four exchanges each 80 us; 8192-key tiles held in
+// registers 144 us -- the few tiles of a small count leave the work to a few
+// CUs; the network as loops with permutes for every lane exchange 23.5 us,
+// unrolled 18.1 us, with DPP and swizzles 13.2 us.)
 template <bool kSys>
 __device__ inline bool ms_less(uint32_t sa, uint64_t ka, uint32_t va, uint32_t sb, uint64_t kb, uint32_t vb)
 {
@@ -841,11 +843,28 @@ __device__ inline bool ms_less(uint32_t sa, uint64_t ka, uint32_t va, uint32_t s
     return ka < kb || (ka == kb && va < vb);
 }
 
+// v of lane (lane ^ mask), for a mask that is a compile-time constant once
+// the caller's loops unroll: distances 1 and 2 by DPP quad permutations,
+// 4 to 16 by swizzles inside each half-wave (bitmask mode: lane ^ mask),
+// 32 by a permute.
+__device__ inline uint32_t lane_xor(uint32_t v, int mask)
+{
+    const int x = static_cast<int>(v);
+    switch (mask) {
+    case 1: return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));   // [1,0,3,2]
+    case 2: return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));   // [2,3,0,1]
+    case 4: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x1F | (4 << 10)));
+    case 8: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x1F | (8 << 10)));
+    case 16: return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(x, 0x1F | (16 << 10)));
+    default: return static_cast<uint32_t>(__shfl_xor(x, mask, 64));
+    }
+}
+
 __device__ inline uint64_t shfl_xor_u64(uint64_t x, int mask)
 {
-    const int lo = __shfl_xor(static_cast<int>(static_cast<uint32_t>(x)), mask, 64);
-    const int hi = __shfl_xor(static_cast<int>(static_cast<uint32_t>(x >> 32)), mask, 64);
-    return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+    const uint32_t lo = lane_xor(static_cast<uint32_t>(x), mask);
+    const uint32_t hi = lane_xor(static_cast<uint32_t>(x >> 32), mask);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 template <bool kSys>
@@ -869,8 +888,12 @@ __device__ inline void tile_sort(const uint64_t* __restrict__ ik, const uint32_t
             sy[i] = 0xffu;
         }
     }
+    // fully unrolled: every partner distance is a constant, so the lane
+    // exchanges compile to swizzles and the phase branches disappear
+#pragma unroll
     for (uint32_t k = 2; k <= kMsTile; k <<= 1) {
         const bool asc = (e0 & k) == 0;   // the same for both elements: k >= 2
+#pragma unroll
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             if (j == 1) {
                 const bool sw = asc ? ms_less<kSys>(sy[1], kk[1], vv[1], sy[0], kk[0], vv[0])
@@ -898,8 +921,8 @@ __device__ inline void tile_sort(const uint64_t* __restrict__ ik, const uint32_t
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
                     pk[i] = shfl_xor_u64(kk[i], mask);
-                    pv[i] = static_cast<uint32_t>(__shfl_xor(static_cast<int>(vv[i]), mask, 64));
-                    ps[i] = kSys ? static_cast<uint32_t>(__shfl_xor(static_cast<int>(sy[i]), mask, 64)) : 0u;
+                    pv[i] = lane_xor(vv[i], mask);
+                    ps[i] = kSys ? lane_xor(sy[i], mask) : 0u;
                 }
             } else {
 #pragma unroll
