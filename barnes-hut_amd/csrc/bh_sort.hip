@@ -981,30 +981,66 @@ __global__ __launch_bounds__(kMsNT) void k_ns_tile_sort(const uint64_t* __restri
 }
 
 // Element i of a merge round (runs of length L, count m): its run r within
-// a group of kMsWays runs, and its ranks in the group's other runs by binary
-// searches in lockstep (the loads of one step issued together).  A lower-
-// bound search over len entries ends within floor(log2(len)) + 1 steps, so
-// 32 - clz(L) steps finish every run of the round.
+// a group of kMsWays runs, and its ranks in the group's other runs.  The
+// block first stages kMsSamples evenly spaced samples of every run of its
+// group in LDS (every S = L / kMsSamples-th entry); an element's rank in run
+// q lies within S - 1 entries after the last sample below it, found by
+// lockstep binary searches in LDS, and the last log2(S) steps search those
+// entries in memory (the loads of one step issued together).  Three
+// dependent memory rounds for L = 2048 instead of twelve.
+constexpr uint32_t kMsSamples = 256;
+constexpr int kMsSampleSteps = 9;   // floor(log2(kMsSamples)) + 1
+
+template <bool kSys>
 __device__ inline void merge_one(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
                                  uint64_t* __restrict__ ok, uint32_t* __restrict__ ov, const uint8_t* __restrict__ sysd,
-                                 uint32_t m, uint32_t L, size_t i)
+                                 uint32_t m, uint32_t L, size_t i, uint64_t k, uint32_t v,
+                                 const uint64_t (*sk)[kMsSamples], const uint32_t (*sv)[kMsSamples],
+                                 const uint8_t (*ss)[kMsSamples])
 {
     const size_t span = static_cast<size_t>(L) * kMsWays;
     const size_t gstart = i / span * span;
     const uint32_t r = static_cast<uint32_t>((i - gstart) / L);
-    const uint64_t k = ik[i];
-    const uint32_t v = iv[i];
-    const uint8_t s = sysd ? sysd[v] : 0;
+    const uint32_t S = L / kMsSamples;
+    const uint8_t s = kSys ? sysd[v] : 0;
+    uint32_t slo[kMsWays], slen[kMsWays];
+#pragma unroll
+    for (int q = 0; q < kMsWays; ++q) {
+        slo[q] = 0;
+        slen[q] = (static_cast<uint32_t>(q) == r || gstart + static_cast<size_t>(q) * L >= m) ? 0u : kMsSamples;
+    }
+#pragma unroll
+    for (int it = 0; it < kMsSampleSteps; ++it) {
+#pragma unroll
+        for (int q = 0; q < kMsWays; ++q) {
+            if (slen[q] == 0) continue;
+            const uint32_t half = slen[q] >> 1, p = slo[q] + half;
+            if (ns_less(kSys ? ss[q][p] : 0, sk[q][p], sv[q][p], s, k, v)) {
+                slo[q] = p + 1;
+                slen[q] -= half + 1;
+            } else {
+                slen[q] = half;
+            }
+        }
+    }
+    // c = slo[q] samples lie below the element: its rank in run q is 0 if
+    // c = 0, else in [(c - 1) S + 1, c S] (cut at the run's end)
     size_t lo[kMsWays];
     uint32_t len[kMsWays];
 #pragma unroll
     for (int q = 0; q < kMsWays; ++q) {
         const size_t a = gstart + static_cast<size_t>(q) * L;
         const size_t e = min(a + L, static_cast<size_t>(m));
-        lo[q] = a;
-        len[q] = (static_cast<uint32_t>(q) == r || a >= e) ? 0u : static_cast<uint32_t>(e - a);
+        const uint32_t c = slo[q];
+        if (static_cast<uint32_t>(q) == r || a >= e || c == 0) {
+            lo[q] = a;
+            len[q] = 0;
+        } else {
+            lo[q] = a + static_cast<size_t>(c - 1) * S + 1;
+            len[q] = static_cast<uint32_t>(min(a + static_cast<size_t>(c) * S, e) - lo[q]);
+        }
     }
-    const int steps = 32 - __builtin_clz(L);
+    const int steps = 31 - __builtin_clz(S);   // a window holds <= S - 1 entries
     for (int it = 0; it < steps; ++it) {
         size_t mid[kMsWays];
         uint64_t km[kMsWays];
@@ -1020,7 +1056,7 @@ __device__ inline void merge_one(const uint64_t* __restrict__ ik, const uint32_t
         for (int q = 0; q < kMsWays; ++q) {
             if (len[q] == 0) continue;
             const uint32_t half = len[q] >> 1;
-            if (ns_less(sysd ? sysd[vm[q]] : 0, km[q], vm[q], s, k, v)) {
+            if (ns_less(kSys ? sysd[vm[q]] : 0, km[q], vm[q], s, k, v)) {
                 lo[q] = mid[q] + 1;
                 len[q] -= half + 1;
             } else {
@@ -1038,18 +1074,54 @@ __device__ inline void merge_one(const uint64_t* __restrict__ ik, const uint32_t
 }
 
 // One round: runs of length L in (ik, iv) merged kMsWays at a time into (ok, ov).
+// A block's 256 elements lie in one group (L >= 2048): the block stages that
+// group's samples, padded past the end of the keys with an entry above every
+// real one.  A thread loads its first element before the count m arrives
+// (below cap the arrays are allocated; an element at or past m is unused).
 __global__ __launch_bounds__(kNT) void k_ns_merge_round(const uint64_t* __restrict__ ik, const uint32_t* __restrict__ iv,
                                                         uint64_t* __restrict__ ok, uint32_t* __restrict__ ov,
                                                         const uint8_t* __restrict__ sysd, uint32_t* tmp, size_t cap,
                                                         uint32_t L)
 {
+    __shared__ uint64_t sk[kMsWays][kMsSamples];
+    __shared__ uint32_t sv[kMsWays][kMsSamples];
+    __shared__ uint8_t ss[kMsWays][kMsSamples];
     const NsTmp nt = ns_tmp_at(tmp, cap);
+    size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x;
+    uint64_t k = 0;
+    uint32_t v = 0;
+    if (i < cap) {
+        k = ik[i];
+        v = iv[i];
+    }
     const uint32_t m = nt.ctl[1];
     if (L >= m) return;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * kNT + threadIdx.x; i < m; i += static_cast<size_t>(gridDim.x) * kNT)
-        merge_one(ik, iv, ok, ov, sysd, m, L, i);
+    const uint32_t S = L / kMsSamples;
+    const size_t span = static_cast<size_t>(L) * kMsWays;
+    const size_t stride = static_cast<size_t>(gridDim.x) * kNT;
+    for (size_t base = static_cast<size_t>(blockIdx.x) * kNT; base < m; base += stride, i += stride) {
+        const size_t gstart = base / span * span;
+        __syncthreads();   // the previous pass's searches are done
+        for (uint32_t e = threadIdx.x; e < kMsWays * kMsSamples; e += kNT) {
+            const uint32_t q = e / kMsSamples, p = e % kMsSamples;
+            const size_t x = gstart + static_cast<size_t>(q) * L + static_cast<size_t>(p) * S;
+            const bool in = x < m;
+            const uint32_t vx = in ? iv[x] : ~0u;
+            sk[q][p] = in ? ik[x] : ~0ull;
+            sv[q][p] = vx;
+            ss[q][p] = in ? (sysd ? sysd[vx] : 0) : 0xff;
+        }
+        __syncthreads();
+        if (i < m) {
+            if (base != static_cast<size_t>(blockIdx.x) * kNT) {   // a later pass
+                k = ik[i];
+                v = iv[i];
+            }
+            if (sysd) merge_one<true>(ik, iv, ok, ov, sysd, m, L, i, k, v, sk, sv, ss);
+            else merge_one<false>(ik, iv, ok, ov, sysd, m, L, i, k, v, sk, sv, ss);
+        }
+    }
 }
-
 
 // Merge of the kept run A = ((k1, v1) + R, K keys) and the sorted removed run
 // B (R keys, in (k1, v1) or (k2, v2) by merge_rounds(R)) into (k0, v0); full
