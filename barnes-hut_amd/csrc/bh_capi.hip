@@ -1145,17 +1145,23 @@ int bh_autotune(bh_ctx* c, int form, int reps, bh_tune_result* out)
                 cands.push_back({c->gp, bl, ms});
                 if (tried == 1 || ms < best_ms) best_ms = ms;
             }
-    // among the candidates within 1% of the fastest (timing noise), the
-    // largest groups, then the full walk grid, then the widest extent limit:
-    // on the 300k system a 48-body limit, and an extent factor of 2 against
-    // 4, timed within noise of the other and then stepped 4% and 0.6% slower
+    // The largest group limit among the candidates within 3% of the fastest,
+    // then, among that limit's candidates within 1% of its fastest (timing
+    // noise), the full walk grid, then the widest extent limit.  On the 300k
+    // system a 48-body limit timed 1-2% faster than 64 in this sweep and then
+    // stepped 4-5% slower (walk 0.419 -> 0.441 ms), and an extent factor of 2
+    // against 4 timed within noise of the other and stepped 0.6% slower.
+    uint32_t gbig = 0;
+    for (const Cand& k : cands)
+        if (k.ms <= best_ms * 1.03 && k.gp.gmax > gbig) gbig = k.gp.gmax;
+    double gfast = 0.0;
+    for (const Cand& k : cands)
+        if (k.gp.gmax == gbig && (gfast == 0.0 || k.ms < gfast)) gfast = k.ms;
     bool have = false;
     for (const Cand& k : cands) {
-        if (k.ms > best_ms * 1.01) continue;
+        if (k.gp.gmax != gbig || k.ms > gfast * 1.01) continue;
         const bool better =
-            !have || k.gp.gmax > best.gmax ||
-            (k.gp.gmax == best.gmax &&
-             (k.blocks > best_blocks || (k.blocks == best_blocks && k.gp.efac > best.efac)));
+            !have || k.blocks > best_blocks || (k.blocks == best_blocks && k.gp.efac > best.efac);
         if (better) {
             best = k.gp;
             best_blocks = k.blocks;

@@ -325,7 +325,9 @@ int bh_get_group_params(bh_ctx *ctx, int *gmax, double *efac, int *occ, int *wal
 /* Autotuner of the mixed walk: times `reps` force evaluations of `form` on
  * the current state (ctx's theta, G, soft2; no kick or drift) for each
  * candidate group shape (gmax 32/48/64 x efac 1/2/4/8) and walk occupancy
- * (full, half), keeps the fastest and reports it.  A LEARNED request before
+ * (full, half), keeps the largest gmax among the candidates within 3% of the
+ * fastest and, for it, the fastest within 1% (full grid, then the widest efac,
+ * on ties), and reports it.  A LEARNED request before
  * the model is trained is tuned as MIXED with the context's schedule.
  * Call before the first step: the accelerations of the last candidate stay
  * on the device. */
