@@ -1027,11 +1027,14 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
     const uint32_t a = t % 3, k = (t / 3) % kMaxHeavy, q = t / (3 * kMaxHeavy);
     if (q >= tb.n_sys) return;
     const SysScalars* sy = tb.sys + q;
-    if (k >= sy->n_heavy) return;
-    const uint32_t h = sy->heavy_slot[k];
+    // the system's fields in one round (heavy_slot has kMaxHeavy entries)
+    const uint32_t nh = sy->n_heavy, h = sy->heavy_slot[k], item0 = sy->heavy_item, nc = sy->heavy_chunks;
+    if (k >= nh) return;
+    double* accel = a == 0 ? b.ax : (a == 1 ? b.ay : b.az);
+    double* vel = a == 0 ? b.vx : (a == 1 ? b.vy : b.vz);
+    const double v_old = (p.kick && lane == 0) ? vel[h] : 0.0;   // loaded beside the partials
     Neumaier acc;
-    const size_t first = sy->heavy_item + static_cast<size_t>(k) * sy->heavy_chunks;
-    const uint32_t nc = sy->heavy_chunks;
+    const size_t first = item0 + static_cast<size_t>(k) * nc;
     // eight partials loaded per round before they are added (in chunk order)
     for (uint32_t c0 = static_cast<uint32_t>(lane); c0 < nc; c0 += 64 * 8) {
         double v[8];
@@ -1050,10 +1053,8 @@ __global__ __launch_bounds__(kNT) void k_heavy_final(Bodies b, TreeBuffers tb, M
     Neumaier tot;
     for (int l = 0; l < 64; ++l) tot.add(lanes[w][l]);
     const double v = tot.value();
-    double* accel = a == 0 ? b.ax : (a == 1 ? b.ay : b.az);
-    double* vel = a == 0 ? b.vx : (a == 1 ? b.vy : b.vz);
     accel[h] = v;
-    if (p.kick) vel[h] = vel[h] + p.half_dt * v;
+    if (p.kick) vel[h] = v_old + p.half_dt * v;
 }
 
 // ------------------------------------------------------------ schedule
