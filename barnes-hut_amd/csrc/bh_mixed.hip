@@ -166,10 +166,20 @@ __device__ inline void sys_extents(const TreeBuffers& tb, const GroupParams& gp)
     for (uint32_t q = threadIdx.x; q < tb.n_sys; q += kNT) {
         SysScalars* sy = tb.sys + q;
         const size_t ns = sy->body_hi - sy->body_lo;
+        uint32_t dc[kMaxDepth + 1];   // every depth's count in one round of loads
+#pragma unroll
+        for (int e = 0; e <= kMaxDepth; ++e) dc[e] = sy->dcount[e];
         uint32_t cells = 1;
         int d = 0;
-        for (; d < kMaxDepth && static_cast<size_t>(cells) * gp.occ < ns; ++d) cells += sy->dcount[d];
+#pragma unroll
+        for (int e = 0; e < kMaxDepth; ++e) {
+            if (d == e && static_cast<size_t>(cells) * gp.occ < ns) {
+                cells += dc[e];
+                d = e + 1;
+            }
+        }
         sy->emax = gp.efac * sqrt(sy->side2[d]);
+#pragma unroll
         for (int e = 0; e <= kMaxDepth; ++e) sy->dcount[e] = 0;
     }
 }
@@ -653,6 +663,18 @@ struct Neumaier {
     __device__ double value() const { return s + c; }
 };
 
+// 1 / sqrt(x) for x > 0: the hardware estimate refined by two Newton steps
+// y <- y (3/2 - x/2 y^2), each of which squares the relative error, instead
+// of a square root followed by a division (two long dependent sequences)
+__device__ inline double rsqrt_f64(double x)
+{
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - hx * y * y);
+    y = y * (1.5 - hx * y * y);
+    return y;
+}
+
 // Work item w = (system s, heavy rank k, chunk c): heavy body k of system s
 // summed over the c-th run of 1024 bodies of s by one wavefront -- each lane
 // a Neumaier sum over its stride, lane partials added in lane order -- and
@@ -684,7 +706,7 @@ __device__ inline void heavy_item(const Bodies& b, const TreeBuffers& tb, const 
     for (size_t j = j0 + lane; j < j1; j += 64) {
         const double dx = b.px[j] - x, dy = b.py[j] - y, dz = b.pz[j] - z;
         const double r2 = dx * dx + dy * dy + dz * dz + p.soft2;
-        const double invR = 1.0 / sqrt(r2);
+        const double invR = rsqrt_f64(r2);
         const double s = (j != h && r2 > 0.0) ? p.G * b.m[j] * (invR * invR * invR) : 0.0;
         ax.add(s * dx);
         ay.add(s * dy);
