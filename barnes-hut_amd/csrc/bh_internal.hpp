@@ -82,7 +82,7 @@ struct StepScalars {
     uint32_t cost_total;     // sum of the group weights of the split schedule
     uint32_t part_counter[8];   // one work queue per group part (kWalkParts)
     uint32_t heavy_counter;     // heavy-sum work items taken in the walk's tail
-    uint32_t sched_hist[128];   // groups per cost bucket of the heaviest-first order
+    uint32_t sched_hist[8][128];   // groups per cost bucket of the heaviest-first order, one copy per XCD
     uint32_t sched_cursor[128]; // groups placed per bucket so far (both cleared by k_kick_drift_box)
     uint32_t n_deep;            // branching nodes at depth >= TreeBuffers::deep_from (listed in deep)
 };
@@ -348,11 +348,14 @@ __device__ inline uint32_t cost_bucket(float c)
 
 // Adds a block's bucket counts (LDS hist[kSchedBuckets], complete) to the
 // step's counts; every thread of the block calls it.
+// A block adds its bucket counts into the copy of its XCD (block b runs on
+// XCD b mod 8), so the hundreds of blocks' adds to the few busy buckets
+// spread over eight sets of counters; k_sched_scatter sums the copies.
 __device__ inline void sched_hist_add(const TreeBuffers& tb, const uint32_t* hist)
 {
     __syncthreads();
     if (threadIdx.x < kSchedBuckets && hist[threadIdx.x])
-        atomicAdd(&tb.sc->sched_hist[threadIdx.x], hist[threadIdx.x]);
+        atomicAdd(&tb.sc->sched_hist[blockIdx.x & 7u][threadIdx.x], hist[threadIdx.x]);
 }
 struct MixedParams {
     double theta2, G, soft2, half_dt;

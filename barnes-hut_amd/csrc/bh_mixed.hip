@@ -1130,7 +1130,9 @@ __global__ __launch_bounds__(kNT) void k_sched_scatter(size_t n, TreeBuffers tb,
     const uint32_t G = tb.sc->n_groups;
     if (G > max_groups) return;
     if (threadIdx.x < kSchedBuckets) {
-        const uint32_t v = tb.sc->sched_hist[threadIdx.x];
+        uint32_t v = 0;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) v += tb.sc->sched_hist[c][threadIdx.x];
         const uint32_t incl = wave_incl_scan(v);
         if (threadIdx.x == 63) wave0_total = incl;
         first[threadIdx.x] = incl - v;

@@ -307,10 +307,8 @@ __global__ __launch_bounds__(kNT) void k_keys(Bodies b, size_t n, TreeBuffers tb
             sc->n_deep = 0;
         }
         if (threadIdx.x < kWalkParts) sc->part_counter[threadIdx.x] = 0;
-        if (threadIdx.x < kSchedBuckets) {
-            sc->sched_hist[threadIdx.x] = 0;
-            sc->sched_cursor[threadIdx.x] = 0;
-        }
+        for (uint32_t k = threadIdx.x; k < 8u * kSchedBuckets; k += kNT) (&sc->sched_hist[0][0])[k] = 0;
+        if (threadIdx.x < kSchedBuckets) sc->sched_cursor[threadIdx.x] = 0;
     }
     __syncthreads();
     const size_t i = i0 + threadIdx.x;
