@@ -348,26 +348,43 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuf
     }
     const uint32_t M = tb.sc->n_leaves, B = tb.sc->n_branch;
     if (t >= static_cast<size_t>(M) + B) return;
+    // every load that needs no loaded value first, in one round: the light
+    // moments, box and slot, and the leaf's body range or the node's children
     const double* q = tb.lraw + 4 * t;
-    const double m = q[0];
+    const double m = q[0], q1 = q[1], q2 = q[2], q3 = q[3], sec = tb.lsec[t];
+    double bx[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) bx[a] = tb.lbox[6 * t + a];
+    const uint32_t slot = mb.slot[t];
+    const bool leaf = t < M;
+    uint32_t f0 = 0, f1 = 0, base = 0;
+    int4 ca = make_int4(-1, -1, -1, -1), cb = ca;
+    if (leaf) {
+        f0 = tb.leaf_first[t];
+        f1 = tb.leaf_first[t + 1];
+    } else {
+        const int4* c4 = reinterpret_cast<const int4*>(tb.child + 8 * (t - M));
+        ca = c4[0];
+        cb = c4[1];
+        base = mb.base[t - M];
+    }
     MNode nd;
     if (m != 0.0) {
-        nd.x = q[1] / m;
-        nd.y = q[2] / m;
-        nd.z = q[3] / m;
+        nd.x = q1 / m;
+        nd.y = q2 / m;
+        nd.z = q3 / m;
     } else {
         nd.x = nd.y = nd.z = 0.0;
     }
     nd.gm = static_cast<float>(G * m);
     // Q = sum m |x - com|^2 = sum m |x|^2 - |sum m x|^2 / m; the cancellation
     // costs (|x| / extent)^2 ulps, harmless for an opening test
-    const double quad = m != 0.0 ? tb.lsec[t] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / m : 0.0;
+    const double quad = m != 0.0 ? sec - (q1 * q1 + q2 * q2 + q3 * q3) / m : 0.0;
     nd.gq = static_cast<float>(G * (quad > 0.0 ? quad : 0.0));
     // opening size: the farthest corner of the light bodies' box seen from the
     // centre of mass (0 for an empty node)
     double s2 = 0.0;
     if (m != 0.0) {
-        const double* bx = tb.lbox + 6 * t;
         const double c3[3] = {nd.x, nd.y, nd.z};
         for (int a = 0; a < 3; ++a) {
             const double e = fmax(c3[a] - bx[a], bx[3 + a] - c3[a]);
@@ -375,27 +392,26 @@ __global__ __launch_bounds__(kNT) void k_mixed_nodes(size_t n, Bodies b, TreeBuf
         }
     }
     nd.size2 = s2;
-    if (t < M) {
-        nd.first = tb.leaf_first[t];
-        nd.info = (tb.leaf_first[t + 1] - tb.leaf_first[t]) | kMLeaf;
-        if (tb.leaf_first[t + 1] - tb.leaf_first[t] == 1u) {
+    if (leaf) {
+        nd.first = f0;
+        nd.info = (f1 - f0) | kMLeaf;
+        if (f1 - f0 == 1u) {
             // the body itself (its light mass is its mass, or 0 if heavy):
             // the walk appends it from the node alone, with the very offsets
             // a body entry gives, which the self-term test relies on
-            nd.x = b.px[nd.first];
-            nd.y = b.py[nd.first];
-            nd.z = b.pz[nd.first];
+            nd.x = b.px[f0];
+            nd.y = b.py[f0];
+            nd.z = b.pz[f0];
         }
     } else {
-        const uint32_t bi = static_cast<uint32_t>(t - M);
-        uint32_t c = 0;
-        const int32_t* ch = tb.child + 8 * static_cast<size_t>(bi);
+        const int32_t c[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+        uint32_t k = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) c += ch[k] >= 0 ? 1u : 0u;
-        nd.first = tb.n_sys + mb.base[bi];
-        nd.info = c;
+        for (int o = 0; o < 8; ++o) k += c[o] >= 0 ? 1u : 0u;
+        nd.first = tb.n_sys + base;
+        nd.info = k;
     }
-    mb.node[mb.slot[t]] = nd;
+    mb.node[slot] = nd;
 }
 
 // ------------------------------------------------------------- walk

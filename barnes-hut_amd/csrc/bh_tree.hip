@@ -79,11 +79,20 @@ __device__ inline int split_depth(uint64_t a, uint64_t b)
 __device__ inline double dmin(double a, double b) { return (b < a) ? b : a; }
 __device__ inline double dmax(double a, double b) { return (a < b) ? b : a; }
 
-__device__ inline uint32_t leaf_sys(const TreeBuffers& tb, uint32_t j) { return tb.bsys[tb.leaf_first[j]]; }
+// (with one system no load: the kernels then skip two dependent memory trips)
+__device__ inline uint32_t leaf_sys(const TreeBuffers& tb, uint32_t j)
+{
+    return tb.n_sys == 1 ? 0u : tb.bsys[tb.leaf_first[j]];
+}
 
 // leaves [lo, hi) of system s
 __device__ inline void sys_leaves(const TreeBuffers& tb, uint32_t M, uint32_t s, uint32_t& lo, uint32_t& hi)
 {
+    if (tb.n_sys == 1) {   // the only system starts at leaf 0
+        lo = 0;
+        hi = M;
+        return;
+    }
     lo = tb.sys[s].leaf_lo;
     hi = (s + 1 < tb.n_sys) ? tb.sys[s + 1].leaf_lo : M;
 }
